@@ -1,0 +1,288 @@
+// api.cpp -- the C ABI of include/shg.h: parameters, handles, dispatch.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "runtime.hpp"
+
+namespace shg {
+const char* last_error();
+int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs);
+int run_plain(shg_handle* h, const shg_params* p, shg_result* r);
+int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r);
+}  // namespace shg
+
+using namespace shg;
+
+extern "C" {
+
+int shg_fft_size_supported(int32_t n) {
+    return n == 64 || n == 128 || n == 192 || n == 256 || n == 384 || n == 512;
+}
+
+int shg_params_set_crystal(shg_params* p, int32_t crystal) {
+    if (!p) return SHG_EINVAL;
+    if (crystal != SHG_CRYSTAL_MGO_SPPLT && crystal != SHG_CRYSTAL_MGO_PPLN) {
+        set_error("unknown crystal id %d", crystal);
+        return SHG_EINVAL;
+    }
+    p->crystal = crystal;
+    if (crystal == SHG_CRYSTAL_MGO_PPLN) {
+        // literature values, no reference counterpart: d33 of MgO:LN
+        // (Shoji et al. 1997), LN conductivity and grating expansion
+        p->deff_um_per_V = 25.0e-6;
+        p->Lambda_um = 6.97;
+        p->k_th = 4.6e-6;
+        p->therm_exp_a = 1.54e-5;
+        p->therm_exp_b = 5.3e-9;
+    } else {
+        // src/headers/Crystal.h:40-52 and src/headers/PhaseMatching.h:53,74
+        p->deff_um_per_V = 11.00e-6;
+        p->Lambda_um = 7.97;
+        p->k_th = 8.4e-6;
+        p->therm_exp_a = 2.2e-6;
+        p->therm_exp_b = -5.9e-9;
+    }
+    p->alpha_p = 0.17e-6;
+    p->alpha_s = 1.57e-6;
+    p->beta_s = 5e-5;
+    p->rho_p = 0.0;
+    p->rho_s = 0.0;
+    p->h_conv = 10e-12;
+    return SHG_OK;
+}
+
+void shg_params_default(shg_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = 128;  // src/cuSHG.cu:11-13
+    p->nz = 100;
+    p->nz_heat = 0;
+    p->lx_um = 2000.0;  // src/headers/Crystal.h:31-33
+    p->ly_um = 1000.0;
+    p->lz_um = 30000.0;
+    shg_params_set_crystal(p, SHG_CRYSTAL_MGO_SPPLT);
+    p->scheme = SHG_SCHEME_REFERENCE;
+    p->lambda_p_um = 1.064;  // src/headers/Crystal.h:26
+    p->power_W = 30.0;       // src/cuSHG.sh:43-45
+    p->waist_um = 29.0;
+    p->focal_um = 0.5 * p->lz_um;  // src/cuSHG.cu:48
+    p->T_C = 55.66;
+    p->T_inf_C = 25.0;  // src/cuSHG.cu:52
+    p->T_peltier_C = 55.66;
+    p->thermal = 0;
+    p->precision = SHG_PREC_FP32_MIXED;
+    p->heat_solver = SHG_HEAT_JACOBI_REFERENCE;
+    p->max_outer = 20;       // src/headers/Solver.h:325
+    p->check_every = 5000;   // src/headers/Solver.h:342
+    p->max_sweeps = 50000000;
+    p->jacobi_tol = 5e-4;    // src/headers/Solver.h:332 (x TSIZE)
+    p->heat_tol_K = 1e-8;
+    p->poling_phase = nullptr;
+}
+
+double shg_refractive_index(int32_t crystal, double lambda_um, double T_C) {
+    return refr_index<double>(crystal, lambda_um, T_C);
+}
+
+double shg_delta_k(const shg_params* p, double T_C) {
+    return delta_k(p->crystal, p->lambda_p_um, p->Lambda_um, p->therm_exp_a, p->therm_exp_b, T_C);
+}
+
+int shg_validate(const shg_params* p) {
+    if (!p) {
+        set_error("null params");
+        return SHG_EINVAL;
+    }
+    if (!shg_fft_size_supported(p->nx) || !shg_fft_size_supported(p->ny)) {
+        set_error("nx=%d ny=%d: each must be one of 64,128,192,256,384,512", p->nx, p->ny);
+        return SHG_EINVAL;
+    }
+    if (p->nz < 2) {
+        set_error("nz=%d must be >= 2", p->nz);
+        return SHG_EINVAL;
+    }
+    const int nzh = p->nz_heat > 0 ? p->nz_heat : p->nz;
+    if (p->thermal && nzh < 2) {
+        set_error("nz_heat=%d must be >= 2", nzh);
+        return SHG_EINVAL;
+    }
+    if (!(p->lx_um > 0) || !(p->ly_um > 0) || !(p->lz_um > 0)) {
+        set_error("window/crystal lengths must be > 0");
+        return SHG_EINVAL;
+    }
+    if (!(p->waist_um > 0) || !(p->power_W >= 0) || !std::isfinite(p->focal_um)) {
+        set_error("waist must be > 0, power >= 0, focus finite");
+        return SHG_EINVAL;
+    }
+    if (!(p->lambda_p_um > 0.3) || !(p->lambda_p_um < 5.0)) {
+        set_error("lambda_p=%g out of range", p->lambda_p_um);
+        return SHG_EINVAL;
+    }
+    if (p->crystal != SHG_CRYSTAL_MGO_SPPLT && p->crystal != SHG_CRYSTAL_MGO_PPLN) {
+        set_error("unknown crystal id %d", p->crystal);
+        return SHG_EINVAL;
+    }
+    if (p->scheme != SHG_SCHEME_REFERENCE && p->scheme != SHG_SCHEME_STRANG) {
+        set_error("unknown scheme %d", p->scheme);
+        return SHG_EINVAL;
+    }
+    if (p->precision != SHG_PREC_FP32_MIXED && p->precision != SHG_PREC_FP64_PLAIN) {
+        set_error("unknown precision %d", p->precision);
+        return SHG_EINVAL;
+    }
+    if (p->thermal) {
+        if (p->heat_solver != SHG_HEAT_JACOBI_REFERENCE && p->heat_solver != SHG_HEAT_DIRECT) {
+            set_error("unknown heat solver %d", p->heat_solver);
+            return SHG_EINVAL;
+        }
+        if (!(p->k_th > 0) || !(p->h_conv >= 0) || !(p->jacobi_tol > 0)) {
+            set_error("k_th must be > 0, h_conv >= 0, jacobi_tol > 0");
+            return SHG_EINVAL;
+        }
+    }
+    const double vals[] = {p->T_C, p->T_inf_C, p->T_peltier_C, p->alpha_p, p->alpha_s,
+                           p->beta_s, p->deff_um_per_V, p->Lambda_um, p->rho_p, p->rho_s};
+    for (double v : vals)
+        if (!std::isfinite(v)) {
+            set_error("non-finite parameter");
+            return SHG_EINVAL;
+        }
+    if (!(p->Lambda_um > 0)) {
+        set_error("Lambda must be > 0");
+        return SHG_EINVAL;
+    }
+    return SHG_OK;
+}
+
+const char* shg_last_error(void) { return last_error(); }
+
+int shg_handle_create(int32_t device, shg_handle** out) {
+    if (!out) return SHG_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_error("no HIP device available");
+        return SHG_EDEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        set_error("device %d out of range (%d devices)", device, ndev);
+        return SHG_EINVAL;
+    }
+    SHG_HIP(hipSetDevice(device));
+    shg_handle* h = new shg_handle();
+    h->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->d_scalars), sizeof(double) * 64);
+    if (e != hipSuccess) {
+        set_error("handle create: %s", hipGetErrorString(e));
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return SHG_EDEVICE;
+    }
+    *out = h;
+    return SHG_OK;
+}
+
+int shg_handle_destroy(shg_handle* h) {
+    if (!h) return SHG_OK;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    prof_collect(h);
+    for (auto& kv : h->tw32) (void)hipFree(kv.second);
+    for (auto& kv : h->tw64) (void)hipFree(kv.second);
+    for (auto e : h->event_pool) (void)hipEventDestroy(e);
+    if (h->arena.base) (void)hipFree(h->arena.base);
+    if (h->d_scalars) (void)hipFree(h->d_scalars);
+    if (h->d_partials) (void)hipFree(h->d_partials);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SHG_OK;
+}
+
+int shg_run_batch(shg_handle* h, const shg_params* p, int32_t n, shg_result* r) {
+    if (!h || !p || !r || n < 0) {
+        set_error("shg_run_batch: null argument");
+        return SHG_EINVAL;
+    }
+    clear_error();
+    SHG_HIP(hipSetDevice(h->device));
+    int first_fail = SHG_OK;
+    std::vector<const shg_params*> mixed;
+    std::vector<shg_result*> mixed_r;
+    for (int i = 0; i < n; i++) {
+        r[i].status = shg_validate(&p[i]);
+        if (r[i].status != SHG_OK) {
+            if (first_fail == SHG_OK) first_fail = r[i].status;
+            continue;
+        }
+        if (p[i].precision == SHG_PREC_FP64_PLAIN) {
+            const int rc = run_plain(h, &p[i], &r[i]);
+            r[i].status = rc;
+            if (rc != SHG_OK && first_fail == SHG_OK) first_fail = rc;
+        } else {
+            mixed.push_back(&p[i]);
+            mixed_r.push_back(&r[i]);
+        }
+    }
+    if (!mixed.empty()) {
+        const int rc = run_mixed(h, mixed, mixed_r);
+        for (auto* rr : mixed_r)
+            if (rr->status != SHG_OK && first_fail == SHG_OK) first_fail = rr->status;
+        if (rc != SHG_OK && first_fail == SHG_OK) first_fail = rc;
+    }
+    return first_fail;
+}
+
+int shg_run(shg_handle* h, const shg_params* p, shg_result* r) {
+    if (!r) {
+        set_error("shg_run: null result");
+        return SHG_EINVAL;
+    }
+    return shg_run_batch(h, p, 1, r);
+}
+
+int shg_adjoint(shg_handle* h, const shg_params* p, int32_t checkpoint_every, double* d_eta,
+                shg_result* r) {
+    if (!h || !p || !d_eta) {
+        set_error("shg_adjoint: null argument");
+        return SHG_EINVAL;
+    }
+    clear_error();
+    int rc = shg_validate(p);
+    if (rc) return rc;
+    SHG_HIP(hipSetDevice(h->device));
+    return run_adjoint(h, p, checkpoint_every, d_eta, r);
+}
+
+int shg_last_kernel_times(shg_handle* h, const char** names, double* ms, int32_t* launches,
+                          int32_t cap) {
+    if (!h) return 0;
+    prof_collect(h);
+    h->knames.clear();
+    for (auto& kv : h->ktimes) h->knames.push_back(kv.first);
+    int i = 0;
+    for (auto& kv : h->ktimes) {
+        if (i >= cap) break;
+        if (names) names[i] = h->knames[i].c_str();
+        if (ms) ms[i] = kv.second.ms;
+        if (launches) launches[i] = kv.second.launches;
+        i++;
+    }
+    return i;
+}
+
+int shg_handle_set_profiling(shg_handle* h, int32_t on) {
+    if (!h) return SHG_EINVAL;
+    prof_reset(h);
+    h->profiling = on != 0;
+    return SHG_OK;
+}
+
+int shg_set_plan(shg_handle* h, int32_t n, int32_t rows_per_wg, int32_t cols_per_wg);
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// common.hpp -- shared host/device helpers of the SHG library (gfx950).
+//
+// Complex values are float2 / double2 (interleaved re, im).  The complex
+// algebra replaces src/headers/Operators.h:13-192 of the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/shg.h"
+
+namespace shg {
+
+// src/headers/Crystal.h:19-21 (um, ps, W, V)
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kC = 299792458.0 * 1e6 / 1e12;
+constexpr double kEps0 = 8.8541878128e-12 * 1e12 / 1e6;
+
+template <typename T> struct cpx_t;
+template <> struct cpx_t<float> { using type = float2; };
+template <> struct cpx_t<double> { using type = double2; };
+template <typename T> using cpx = typename cpx_t<T>::type;
+
+template <typename C> __host__ __device__ __forceinline__ C mk(decltype(C{}.x) a, decltype(C{}.x) b) {
+    C c;
+    c.x = a;
+    c.y = b;
+    return c;
+}
+// Complex product.  (The HIP vector types' own operator* is elementwise;
+// +, - and scalar * of those types are the complex ones.)
+template <typename C> __host__ __device__ __forceinline__ C cmul(C a, C b) {
+    C c;
+    c.x = a.x * b.x - a.y * b.y;
+    c.y = a.x * b.y + a.y * b.x;
+    return c;
+}
+template <typename C> __host__ __device__ __forceinline__ C conjc(C a) { a.y = -a.y; return a; }
+// multiply by +i or -i
+template <typename C> __host__ __device__ __forceinline__ C mul_i(C a) { C r; r.x = -a.y; r.y = a.x; return r; }
+template <typename C> __host__ __device__ __forceinline__ C mul_mi(C a) { C r; r.x = a.y; r.y = -a.x; return r; }
+template <typename C> __host__ __device__ __forceinline__ decltype(C{}.x) abs2(C a) { return a.x * a.x + a.y * a.y; }
+
+// Extraordinary index n(lambda, T).  MgO:sPPLT per src/headers/Crystal.h:54-70
+// (Bruner et al.); MgO:PPLN per Gayer et al., Appl. Phys. B 91, 343 (2008).
+template <typename R>
+__host__ __device__ __forceinline__ R refr_index(int crystal, R L, R T) {
+    const R L2 = L * L;
+    if (crystal == SHG_CRYSTAL_MGO_PPLN) {
+        const R f = (T - R(24.5)) * (T + R(570.82));
+        const R g = R(0.2020) + R(6.113e-8) * f;
+        const R n2 = R(5.756) + R(2.860e-6) * f + (R(0.0983) + R(4.700e-8) * f) / (L2 - g * g) +
+                     (R(189.32) + R(1.516e-4) * f) / (L2 - R(12.52) * R(12.52)) - R(1.32e-2) * L2;
+        return sqrt(n2);
+    }
+    const R tk = T + R(273.15);
+    const R b = R(3.483933e-8) * tk * tk;
+    const R c = R(1.607839e-8) * tk * tk;
+    const R cc = R(0.185087) + c;
+    const R n2 = R(4.502483) + (R(0.007294) + b) / (L2 - cc * cc) +
+                 R(0.073423) / (L2 - R(0.199595) * R(0.199595)) +
+                 R(0.001) / (L2 - R(7.99724) * R(7.99724)) + R(-0.02357) * L2;
+    return sqrt(n2);
+}
+
+// dk(T) [1/um] in FP64 -- src/headers/PhaseMatching.h:59,80
+__host__ __device__ __forceinline__ double delta_k(int crystal, double lp, double Lambda, double ea,
+                                                   double eb, double T) {
+    const double ls = 0.5 * lp;
+    const double np = refr_index<double>(crystal, lp, T);
+    const double ns = refr_index<double>(crystal, ls, T);
+    const double d = T - 25.0;
+    return 2.0 * kPi * (2.0 * np / lp - ns / ls + 1.0 / (Lambda * (1.0 + ea * d + eb * d * d)));
+}
+
+// Phase reduced to [-pi, pi] in FP64 before the FP32 sincos.
+__device__ __forceinline__ float2 expi_reduced(double ph) {
+    const double twopi = 2.0 * kPi;
+    const double r = ph - twopi * rint(ph * (1.0 / twopi));
+    float s, c;
+    sincosf((float)r, &s, &c);
+    return make_float2(c, s);
+}
+
+}  // namespace shg

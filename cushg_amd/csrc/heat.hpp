@@ -1,0 +1,41 @@
+// heat.hpp -- THERMAL: steady-state heat solve on the GPU and the optics
+// inputs derived from the temperature (host orchestration).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/shg.h"
+
+struct shg_handle;
+
+namespace shg {
+
+struct HeatState {
+    int nx = 0, ny = 0, nzh = 0;
+    double dx = 0, dy = 0, dz = 0;
+    double* T = nullptr;       // FP64 temperature [nzh][ny][nx]
+    double* Tn = nullptr;      // FP64 scratch
+    float* T32 = nullptr;      // FP32 copy read by the optics
+    float* Q = nullptr;        // heat source [nzh][ny][nx]
+    double* dk = nullptr;      // dk(T) [nzh][ny][nx]
+    double* dkbase = nullptr;  // running-integral base per heat slice
+    double* pol = nullptr;     // poling phases [nz] or null
+    double* red = nullptr;     // reduction slots
+    void* solver = nullptr;    // direct-solver tables (opaque)
+    bool converged_in_one = false;
+};
+
+struct HeatSolveInfo {
+    long long sweeps = 0;
+    double residual = 0;
+    bool converged_in_one = false;
+};
+
+size_t heat_bytes(const shg_params* p);
+int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs);
+int heat_init(shg_handle* h, const shg_params* p, HeatState& hs);
+int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo& info);
+int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc);
+int heat_finish(shg_handle* h, const shg_params* p, HeatState& hs, shg_result* r);
+
+}  // namespace shg

@@ -1,0 +1,57 @@
+// kernels.hpp -- device-side instance descriptors and launch entry points.
+#pragma once
+
+#include "common.hpp"
+
+namespace shg {
+
+// One instance of the FP32-mixed march, as the kernels see it.  Lives in
+// device memory (an array per launch group, indexed by blockIdx.z).
+struct MixedInst {
+    float2* P;              // pump field [ny][nx]
+    float2* S;              // SH field   [ny][nx]
+    float2* outP;           // state after nz-1 slices (the reference's Pump_out)
+    float2* outS;
+    const double2* exP;     // separable propagator, x factor [nx] (includes 1/(nx ny))
+    const double2* exS;
+    const double2* eyP;     // y factor [ny]
+    const double2* eyS;
+    const float2* phase;    // non-THERMAL: exp(i phi) [nz][2 calls][3 offsets]
+    const float* T;         // THERMAL: temperature [nzh][ny][nx]
+    const double* dkbase;   // THERMAL: running dk integral = dkbase[h] + s*dk[h]
+    const double* dk;       // THERMAL: dk(T) [nzh][ny][nx]
+    float* Q;               // THERMAL: heat source [nzh][ny][nx]
+    const double* poling;   // THERMAL: per-slice poling phase [nz] or null
+    double dz;
+    int nx, ny, nz, nzh;
+    int crystal;
+    float lp, ls;
+    float kp, ks, bfac;     // non-THERMAL couplings at T_C
+    float kp0, ks0, bf0;    // THERMAL: kp = kp0/n_p(T), ks = ks0/n_s(T), bfac = bf0 n_s(T)
+    float ap_h, as_h;       // alpha/2
+    float c1, c2, c3, w;    // RK4 stage coefficients and final weight
+    float off[2][3];        // z offsets (units of dz) per call
+    float hce;              // c eps0 / 2
+    float alpha_p, alpha_s, beta_s;
+};
+
+// FP32-mixed pipeline (kernels_mixed.hip)
+hipError_t launch_rowpass(int nx, bool thermal, int rows_pw, dim3 grid, hipStream_t st,
+                          const MixedInst* insts, int g, const float4* tw);
+hipError_t launch_colpass(int ny, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
+                          int g, const float4* tw);
+int rowpass_rows_default(int nx);
+int colpass_cols_default(int ny);
+bool colpass_cols_supported(int ny, int cols);
+bool rowpass_rows_supported(int nx, int rows);
+
+// utility kernels (kernels_util.hip)
+template <typename C>
+hipError_t launch_pump_init(hipStream_t st, C* P, int nx, int ny, double dx, double dy, double w2,
+                            double Ap0, double Mre, double Mim);
+hipError_t launch_sumsq_f32(hipStream_t st, const float2* A, long n, double* partials, int nblocks);
+hipError_t launch_sumsq_f64(hipStream_t st, const double2* A, long n, double* partials, int nblocks);
+hipError_t launch_sum_partials(hipStream_t st, const double* partials, int n, double* out);
+hipError_t launch_f64_to_f32(hipStream_t st, const double* in, float* out, long n);
+
+}  // namespace shg

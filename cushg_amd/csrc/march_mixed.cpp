@@ -1,0 +1,455 @@
+// march_mixed.cpp -- host orchestration of the FP32-mixed form: instance
+// set-up, the batched z-march (Solver::CWES, src/headers/Solver.h:276-308)
+// and, with THERMAL, the thermo-optic outer loop (src/headers/Solver.h:321-373).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+#include "heat.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace shg {
+
+struct MixedHost {
+    const shg_params* p = nullptr;
+    shg_result* r = nullptr;
+    int nx = 0, ny = 0, nz = 0, nzh = 0;
+    size_t N2 = 0;
+    double dx = 0, dy = 0, dz = 0;
+    double Tc = 0;  // temperature at the grid centre of heat slice 0
+    float2 *P = nullptr, *S = nullptr, *outP = nullptr, *outS = nullptr;
+    double2 *exP = nullptr, *exS = nullptr, *eyP = nullptr, *eyS = nullptr;
+    float2* phase = nullptr;
+    double* pol = nullptr;
+    double* slots = nullptr;  // [0] P_in, [1] P_out, [2] P_sh
+    HeatState heat;
+    MixedInst desc{};
+    bool thermal = false;
+    bool running = true;  // THERMAL: still in the outer loop
+    int outer = 0;
+    long long steps = 0, sweeps = 0;
+    double heat_res = 0;
+};
+
+static const char* rowpass_name(int nx, bool th) {
+    static std::map<std::pair<int, bool>, std::string> names;
+    auto& s = names[{nx, th}];
+    if (s.empty()) s = std::string("rowpass_") + std::to_string(nx) + (th ? "_thermal" : "");
+    return s.c_str();
+}
+static const char* colpass_name(int ny) {
+    static std::map<int, std::string> names;
+    auto& s = names[ny];
+    if (s.empty()) s = "colpass_" + std::to_string(ny);
+    return s.c_str();
+}
+
+int plan_rows(shg_handle* h, int nx) {
+    auto it = h->rows_plan.find(nx);
+    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx);
+}
+int plan_cols(shg_handle* h, int ny) {
+    auto it = h->cols_plan.find(ny);
+    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_default(ny);
+}
+
+// Host-side FP64 tables of one instance at centre temperature Tc:
+// separable propagator (src/headers/Efields.h:122-141 with the fftshift of
+// :68-81 folded into the index map, and the 1/(nx ny) of cuFFTscale folded
+// into the x factor), per-slice QPM phases, couplings and RK4 constants.
+static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>& ex_s,
+                        std::vector<double2>& ey_p, std::vector<double2>& ey_s,
+                        std::vector<float2>& phase) {
+    const shg_params* p = m.p;
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const double lp = p->lambda_p_um, ls = 0.5 * lp;
+    const double np = refr_index<double>(p->crystal, lp, m.Tc);
+    const double ns = refr_index<double>(p->crystal, ls, m.Tc);
+    const double kwp = 2 * kPi * np / lp, kws = 2 * kPi * ns / ls;
+    const double dfX = 1.0 / m.dx / nx, dfY = 1.0 / m.dy / ny;
+    const double scale = 1.0 / ((double)nx * ny);
+    ex_p.resize(nx);
+    ex_s.resize(nx);
+    ey_p.resize(ny);
+    ey_s.resize(ny);
+    for (int x = 0; x < nx; x++) {
+        const int ic = (x + nx / 2) % nx;
+        const double fx = (ic - 0.5 * nx) * dfX;
+        const double ap = -m.dz * (2 * kPi * kPi / kwp * fx * fx + 2 * kPi * fx * std::tan(p->rho_p));
+        const double as = -m.dz * (2 * kPi * kPi / kws * fx * fx + 2 * kPi * fx * std::tan(p->rho_s));
+        ex_p[x] = make_double2(scale * std::cos(ap), scale * std::sin(ap));
+        ex_s[x] = make_double2(scale * std::cos(as), scale * std::sin(as));
+    }
+    for (int y = 0; y < ny; y++) {
+        const int jc = (y + ny / 2) % ny;
+        const double fy = (jc - 0.5 * ny) * dfY;
+        const double ap = -m.dz * (2 * kPi * kPi / kwp * fy * fy);
+        const double as = -m.dz * (2 * kPi * kPi / kws * fy * fy);
+        ey_p[y] = make_double2(std::cos(ap), std::sin(ap));
+        ey_s[y] = make_double2(std::cos(as), std::sin(as));
+    }
+    // scheme constants (src/headers/Solver.h:235-256 or textbook Strang)
+    MixedInst& d = m.desc;
+    double kmul = 1.0;
+    double off[2][3] = {{0.0, 0.25, 0.5}, {0.0, 0.25, 0.5}};
+    if (p->scheme == SHG_SCHEME_STRANG) {
+        const double hh = 0.5 * m.dz;
+        d.c1 = (float)(0.5 * hh);
+        d.c2 = (float)(0.5 * hh);
+        d.c3 = (float)hh;
+        d.w = (float)(hh / 6.0);
+        kmul = 2.0;
+        off[1][0] = 0.5;
+        off[1][1] = 0.75;
+        off[1][2] = 1.0;
+    } else {
+        d.c1 = 0.5f;
+        d.c2 = 0.5f;
+        d.c3 = 1.0f;
+        d.w = (float)(m.dz / 6.0);
+    }
+    for (int c = 0; c < 2; c++)
+        for (int i = 0; i < 3; i++) d.off[c][i] = (float)off[c][i];
+    const double dQ = 2.0 * p->deff_um_per_V / kPi;
+    d.kp = (float)(kmul * kPi * dQ / (np * lp));
+    d.ks = (float)(kmul * 0.5 * kPi * dQ / (ns * ls));
+    d.bfac = (float)(p->beta_s * 0.5 * kC * kEps0 * ns);
+    d.kp0 = (float)(kmul * kPi * dQ / lp);
+    d.ks0 = (float)(kmul * 0.5 * kPi * dQ / ls);
+    d.bf0 = (float)(p->beta_s * 0.5 * kC * kEps0);
+    d.ap_h = (float)(0.5 * p->alpha_p);
+    d.as_h = (float)(0.5 * p->alpha_s);
+    d.hce = (float)(0.5 * kC * kEps0);
+    d.alpha_p = (float)p->alpha_p;
+    d.alpha_s = (float)p->alpha_s;
+    d.beta_s = (float)p->beta_s;
+    d.crystal = p->crystal;
+    d.lp = (float)lp;
+    d.ls = (float)ls;
+    d.dz = m.dz;
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.nzh = m.nzh;
+    // non-THERMAL phase table: exp(i ((s + off) dz dk + poling[s])), FP64
+    if (!m.thermal) {
+        const double dk = delta_k(p->crystal, lp, p->Lambda_um, p->therm_exp_a, p->therm_exp_b, m.Tc);
+        phase.resize((size_t)nz * 6);
+        for (int s = 0; s < nz; s++) {
+            const double pol = p->poling_phase ? p->poling_phase[s] : 0.0;
+            for (int c = 0; c < 2; c++)
+                for (int i = 0; i < 3; i++) {
+                    const double ph = ((double)s + off[c][i]) * m.dz * dk + pol;
+                    phase[(size_t)(s * 2 + c) * 3 + i] = make_float2((float)std::cos(ph), (float)std::sin(ph));
+                }
+        }
+    } else {
+        phase.clear();
+    }
+}
+
+// Upload tables, initialise the pump (src/headers/Efields.h:84-119) and a
+// zero SH field, and reduce the initial pump power.
+static int setup_fields(shg_handle* h, MixedHost& m) {
+    std::vector<double2> exp_, exs, eyp, eys;
+    std::vector<float2> ph;
+    host_tables(m, exp_, exs, eyp, eys, ph);
+    SHG_HIP(hipMemcpyAsync(m.exP, exp_.data(), sizeof(double2) * m.nx, hipMemcpyHostToDevice, h->stream));
+    SHG_HIP(hipMemcpyAsync(m.exS, exs.data(), sizeof(double2) * m.nx, hipMemcpyHostToDevice, h->stream));
+    SHG_HIP(hipMemcpyAsync(m.eyP, eyp.data(), sizeof(double2) * m.ny, hipMemcpyHostToDevice, h->stream));
+    SHG_HIP(hipMemcpyAsync(m.eyS, eys.data(), sizeof(double2) * m.ny, hipMemcpyHostToDevice, h->stream));
+    if (!ph.empty())
+        SHG_HIP(hipMemcpyAsync(m.phase, ph.data(), sizeof(float2) * ph.size(), hipMemcpyHostToDevice, h->stream));
+    // the host vectors die at return: make the copies complete first
+    SHG_HIP(hipStreamSynchronize(h->stream));
+    const shg_params* p = m.p;
+    const double np = refr_index<double>(p->crystal, p->lambda_p_um, m.Tc);
+    const double w2 = p->waist_um * p->waist_um;
+    const double zR = kPi * np * w2 / p->lambda_p_um;
+    const double Ap0 = std::sqrt(4.0 * p->power_W / (kEps0 * kC * kPi * np * w2));
+    {
+        ProfScope ps(h, "pump_init");
+        SHG_HIP(launch_pump_init<float2>(h->stream, m.P, m.nx, m.ny, m.dx, m.dy, w2, Ap0, 1.0, -p->focal_um / zR));
+    }
+    SHG_HIP(hipMemsetAsync(m.S, 0, sizeof(float2) * m.N2, h->stream));
+    int rc = sum_sq(h, m.P, (long)m.N2, m.slots + 0);
+    return rc;
+}
+
+// The batched z-march: launches A(0) B(0) A(1) ... B(G-1) A(G), G = max nz.
+static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, size_t pool_cap) {
+    size_t pool_used = 0;
+    struct Group {
+        int key_n;
+        bool thermal;
+        std::vector<MixedHost*> m;
+        MixedInst* d = nullptr;
+        const float4* tw = nullptr;
+    };
+    std::map<std::pair<int, bool>, Group> rowg;
+    std::map<int, Group> colg;
+    for (auto* m : act) {
+        auto& rg = rowg[{m->nx, m->thermal}];
+        rg.key_n = m->nx;
+        rg.thermal = m->thermal;
+        rg.m.push_back(m);
+        auto& cg = colg[m->ny];
+        cg.key_n = m->ny;
+        cg.m.push_back(m);
+    }
+    auto by_nz = [](const MixedHost* a, const MixedHost* b) { return a->nz > b->nz; };
+    std::vector<std::vector<MixedInst>> hostdesc;
+    auto upload = [&](Group& g) -> int {
+        std::sort(g.m.begin(), g.m.end(), by_nz);
+        hostdesc.emplace_back();
+        auto& v = hostdesc.back();
+        for (auto* m : g.m) v.push_back(m->desc);
+        g.d = pool_used + v.size() <= pool_cap ? pool + pool_used : nullptr;
+        pool_used += v.size();
+        if (!g.d) {
+            set_error("arena exhausted (descriptors)");
+            return SHG_ENOMEM;
+        }
+        SHG_HIP(hipMemcpyAsync(g.d, v.data(), sizeof(MixedInst) * v.size(), hipMemcpyHostToDevice, h->stream));
+        g.tw = twiddles32(h, g.key_n);
+        if (!g.tw) {
+            set_error("twiddle table allocation failed");
+            return SHG_ENOMEM;
+        }
+        return SHG_OK;
+    };
+    for (auto& kv : rowg)
+        if (int rc = upload(kv.second)) return rc;
+    for (auto& kv : colg)
+        if (int rc = upload(kv.second)) return rc;
+    int G = 0;
+    for (auto* m : act) G = std::max(G, m->nz);
+    for (int g = 0; g <= G; g++) {
+        for (auto& kv : rowg) {
+            Group& gr = kv.second;
+            int cnt = 0, maxny = 0;
+            for (auto* m : gr.m) {
+                if (m->nz < g) break;
+                cnt++;
+                maxny = std::max(maxny, m->ny);
+            }
+            if (!cnt) continue;
+            const int rows = plan_rows(h, gr.key_n);
+            dim3 grid((maxny + rows - 1) / rows, 1, cnt);
+            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal));
+            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, rows, grid, h->stream, gr.d, g, gr.tw));
+        }
+        if (g == G) break;
+        for (auto& kv : colg) {
+            Group& gr = kv.second;
+            int cnt = 0, maxnx = 0;
+            for (auto* m : gr.m) {
+                if (m->nz <= g) break;
+                cnt++;
+                maxnx = std::max(maxnx, m->nx);
+            }
+            if (!cnt) continue;
+            const int cols = plan_cols(h, gr.key_n);
+            dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
+            ProfScope ps(h, colpass_name(gr.key_n));
+            SHG_HIP(launch_colpass(gr.key_n, cols, grid, h->stream, gr.d, g, gr.tw));
+        }
+    }
+    // descriptors' host copies must outlive the async copies
+    SHG_HIP(hipStreamSynchronize(h->stream));
+    for (auto* m : act) m->steps += m->nz;
+    return SHG_OK;
+}
+
+static size_t mixed_bytes(const MixedHost& m) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t b = 4 * al(sizeof(float2) * m.N2) + 2 * al(sizeof(double2) * m.nx) +
+               2 * al(sizeof(double2) * m.ny) + al(sizeof(float2) * (size_t)m.nz * 6) + al(sizeof(double) * 4) +
+               2 * al(sizeof(MixedInst));
+    if (m.thermal) b += heat_bytes(m.p);
+    return b;
+}
+
+int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = (int)ps.size();
+    std::vector<MixedHost> ms(n);
+    size_t total = 0;
+    for (int i = 0; i < n; i++) {
+        MixedHost& m = ms[i];
+        m.p = ps[i];
+        m.r = rs[i];
+        m.nx = m.p->nx;
+        m.ny = m.p->ny;
+        m.nz = m.p->nz;
+        m.nzh = m.p->nz_heat > 0 ? m.p->nz_heat : m.p->nz;
+        m.N2 = (size_t)m.nx * m.ny;
+        m.dx = m.p->lx_um / (m.nx - 1);
+        m.dy = m.p->ly_um / (m.ny - 1);
+        m.dz = m.p->lz_um / (m.nz - 1);
+        m.thermal = m.p->thermal != 0;
+        m.Tc = m.p->T_C;
+        total += mixed_bytes(m);
+    }
+    total += 2 * sizeof(MixedInst) * (size_t)n + 4096;
+    if (int rc = h->arena.reserve(total)) return rc;
+    for (auto& m : ms) {
+        m.P = h->arena.carve<float2>(m.N2);
+        m.S = h->arena.carve<float2>(m.N2);
+        m.outP = h->arena.carve<float2>(m.N2);
+        m.outS = h->arena.carve<float2>(m.N2);
+        m.exP = h->arena.carve<double2>(m.nx);
+        m.exS = h->arena.carve<double2>(m.nx);
+        m.eyP = h->arena.carve<double2>(m.ny);
+        m.eyS = h->arena.carve<double2>(m.ny);
+        m.phase = h->arena.carve<float2>((size_t)m.nz * 6);
+        m.slots = h->arena.carve<double>(4);
+        if (!m.slots) {
+            set_error("arena exhausted");
+            return SHG_ENOMEM;
+        }
+        if (m.thermal) {
+            if (int rc = heat_alloc(h, m.p, m.heat)) return rc;
+        }
+        MixedInst& d = m.desc;
+        d.P = m.P;
+        d.S = m.S;
+        d.outP = m.outP;
+        d.outS = m.outS;
+        d.exP = m.exP;
+        d.exS = m.exS;
+        d.eyP = m.eyP;
+        d.eyS = m.eyS;
+        d.phase = m.phase;
+        if (m.thermal) {
+            d.T = m.heat.T32;
+            d.dkbase = m.heat.dkbase;
+            d.dk = m.heat.dk;
+            d.Q = m.heat.Q;
+            d.poling = m.heat.pol;
+        }
+    }
+
+    MixedInst* pool = h->arena.carve<MixedInst>(2 * (size_t)n);
+    if (!pool) {
+        set_error("arena exhausted (descriptor pool)");
+        return SHG_ENOMEM;
+    }
+    std::vector<MixedHost*> act;
+    for (auto& m : ms)
+        if (!m.thermal) {
+            if (int rc = setup_fields(h, m)) return rc;
+            act.push_back(&m);
+        }
+    // non-THERMAL instances march once; THERMAL ones iterate the outer loop
+    std::vector<MixedHost*> th;
+    for (auto& m : ms)
+        if (m.thermal) {
+            if (int rc = heat_init(h, m.p, m.heat)) return rc;
+            th.push_back(&m);
+        }
+    int max_outer = 0;
+    for (auto* m : th) max_outer = std::max(max_outer, m->p->max_outer > 0 ? m->p->max_outer : 20);
+    for (int go = 0; go <= max_outer; go++) {
+        // heat solve + optics set-up of the THERMAL instances still running
+        for (auto* m : th) {
+            if (!m->running) continue;
+            if (go >= (m->p->max_outer > 0 ? m->p->max_outer : 20)) {
+                m->running = false;
+                continue;
+            }
+            HeatSolveInfo info{};
+            int rc = heat_solve(h, m->p, m->heat, info);
+            if (rc) {
+                m->r->status = rc;
+                m->running = false;
+                continue;
+            }
+            m->sweeps += info.sweeps;
+            m->heat_res = info.residual;
+            m->heat.converged_in_one = info.converged_in_one;
+            rc = heat_prepare_optics(h, m->p, m->heat, &m->Tc);
+            if (rc) return rc;
+            m->desc.T = m->heat.T32;
+            if ((rc = setup_fields(h, *m))) return rc;
+            act.push_back(m);
+        }
+        if (act.empty()) break;
+        if (int rc = march(h, act, pool, 2 * (size_t)n)) return rc;
+        for (auto* m : act) {
+            if (!m->thermal) continue;
+            m->outer = go + 1;
+            // src/headers/Solver.h:367: stop after a pass whose Jacobi loop took one sweep
+            if (m->heat.converged_in_one && go != 0) m->running = false;
+        }
+        // the saved output state of this pass is kept in outP/outS; reduce now
+        for (auto* m : act) {
+            if (int rc = sum_sq(h, m->outP, (long)m->N2, m->slots + 1)) return rc;
+            if (int rc = sum_sq(h, m->outS, (long)m->N2, m->slots + 2)) return rc;
+        }
+        act.clear();
+        if (th.empty()) break;
+        bool any = false;
+        for (auto* m : th) any |= m->running;
+        if (!any) break;
+    }
+
+    // results
+    SHG_HIP(hipStreamSynchronize(h->stream));
+    for (auto& m : ms) {
+        if (m.r->status != SHG_OK) continue;
+        double sl[4];
+        SHG_HIP(hipMemcpy(sl, m.slots, sizeof(double) * 3, hipMemcpyDeviceToHost));
+        const shg_params* p = m.p;
+        const double np = refr_index<double>(p->crystal, p->lambda_p_um, m.Tc);
+        const double ns = refr_index<double>(p->crystal, 0.5 * p->lambda_p_um, m.Tc);
+        const double c = 0.5 * kEps0 * kC * m.dx * m.dy;
+        shg_result* r = m.r;
+        r->P_pump_in_W = c * np * sl[0];
+        r->P_pump_out_W = c * np * sl[1];
+        r->P_sh_W = c * ns * sl[2];
+        r->eta = p->power_W > 0 ? r->P_sh_W / p->power_W : 0.0;
+        r->steps_done = m.steps;
+        r->jacobi_sweeps = m.sweeps;
+        r->outer_iters = m.thermal ? m.outer : 1;
+        r->heat_residual_K = m.heat_res;
+        r->peak_dT_K = 0;
+        r->T_max_C = m.Tc;
+        if (!std::isfinite(sl[0]) || !std::isfinite(sl[1]) || !std::isfinite(sl[2])) {
+            set_error("non-finite field power");
+            r->status = SHG_ENAN;
+            continue;
+        }
+        if (r->pump_out || r->signal_out) {
+            std::vector<float2> buf(m.N2);
+            if (r->pump_out) {
+                SHG_HIP(hipMemcpy(buf.data(), m.outP, sizeof(float2) * m.N2, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < m.N2; i++) {
+                    r->pump_out[2 * i] = buf[i].x;
+                    r->pump_out[2 * i + 1] = buf[i].y;
+                }
+            }
+            if (r->signal_out) {
+                SHG_HIP(hipMemcpy(buf.data(), m.outS, sizeof(float2) * m.N2, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < m.N2; i++) {
+                    r->signal_out[2 * i] = buf[i].x;
+                    r->signal_out[2 * i + 1] = buf[i].y;
+                }
+            }
+        }
+        if (m.thermal) {
+            if (int rc = heat_finish(h, p, m.heat, r)) return rc;
+        }
+        r->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return SHG_OK;
+}
+
+}  // namespace shg

@@ -1,0 +1,91 @@
+// runtime.hpp -- handle, device arena, error and timing helpers (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/shg.h"
+
+namespace shg {
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define SHG_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            ::shg::set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #call,             \
+                             hipGetErrorString(e_));                                   \
+            return e_ == hipErrorOutOfMemory ? SHG_ENOMEM : SHG_EDEVICE;               \
+        }                                                                              \
+    } while (0)
+
+// Bump allocator over one device allocation, grown on demand and reused
+// across calls on the same handle (no per-call hipMalloc once warm).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    int reserve(size_t bytes);  // ensure capacity; invalidates earlier carves
+    void reset() { used = 0; }
+    template <typename T> T* carve(size_t count) {
+        size_t b = (count * sizeof(T) + 255) & ~size_t(255);
+        if (used + b > cap) return nullptr;
+        T* r = reinterpret_cast<T*>(base + used);
+        used += b;
+        return r;
+    }
+};
+
+struct KernelTime {
+    double ms = 0;
+    int launches = 0;
+};
+
+}  // namespace shg
+
+struct shg_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    shg::Arena arena;
+    std::map<int, float4*> tw32;
+    std::map<int, double2*> tw64;
+    double* d_scalars = nullptr;  // reduction slots
+    double* d_partials = nullptr;
+    int n_partials = 0;
+    // tuned launch plans: rows per workgroup (row pass, by nx), columns per
+    // workgroup (column pass, by ny); 0 = built-in default
+    std::map<int, int> rows_plan, cols_plan;
+    // profiling
+    bool profiling = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, shg::KernelTime> ktimes;
+    std::vector<std::string> knames;  // storage for names handed out
+};
+
+namespace shg {
+
+// Profiling wrapper: brackets a launch with events on the handle's stream.
+struct ProfScope {
+    shg_handle* h;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(shg_handle* h_, const char* n);
+    ~ProfScope();
+};
+void prof_collect(shg_handle* h);
+void prof_reset(shg_handle* h);
+
+const float4* twiddles32(shg_handle* h, int n);
+const double2* twiddles64(shg_handle* h, int n);
+
+int sum_sq(shg_handle* h, const float2* A, long n, double* slot);
+int sum_sq64(shg_handle* h, const double2* A, long n, double* slot);
+
+}  // namespace shg

@@ -1,0 +1,23 @@
+// temporary: THERMAL, FP64-plain and adjoint paths not built yet
+#include "heat.hpp"
+#include "runtime.hpp"
+namespace shg {
+size_t heat_bytes(const shg_params*) { return 0; }
+int heat_alloc(shg_handle*, const shg_params*, HeatState&) { set_error("THERMAL not built"); return SHG_EUNSUPPORTED; }
+int heat_init(shg_handle*, const shg_params*, HeatState&) { return SHG_EUNSUPPORTED; }
+int heat_solve(shg_handle*, const shg_params*, HeatState&, HeatSolveInfo&) { return SHG_EUNSUPPORTED; }
+int heat_prepare_optics(shg_handle*, const shg_params*, HeatState&, double*) { return SHG_EUNSUPPORTED; }
+int heat_finish(shg_handle*, const shg_params*, HeatState&, shg_result*) { return SHG_EUNSUPPORTED; }
+int run_plain(shg_handle*, const shg_params*, shg_result*) { set_error("FP64 plain not built"); return SHG_EUNSUPPORTED; }
+int run_adjoint(shg_handle*, const shg_params*, int, double*, shg_result*) { set_error("adjoint not built"); return SHG_EUNSUPPORTED; }
+}
+extern "C" {
+int shg_set_plan(shg_handle* h, int32_t n, int32_t r, int32_t c) {
+    if (!h) return SHG_EINVAL;
+    if (r) h->rows_plan[n] = r;
+    if (c) h->cols_plan[n] = c;
+    return SHG_OK;
+}
+int shg_autotune(shg_handle*, int32_t, int32_t, int32_t) { return SHG_EUNSUPPORTED; }
+int shg_cli_main(int, const char* const*, const shg_params*, const char*) { return SHG_EUNSUPPORTED; }
+}

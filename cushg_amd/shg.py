@@ -1,0 +1,218 @@
+"""ctypes binding of libshg_hip.so (the C ABI of include/shg.h).
+
+``Solver.run`` mirrors the reference's
+``Solver::run(Power, waist, focalpoint, Temp, T_inf, Tpeltier,
+save_only_last, save_temperature)`` (src/headers/Solver.h:149,311): same
+argument names and meaning, results returned instead of only written to
+files.  Errors raise ``ShgError`` carrying the C status code; the library
+never exits the process (the reference's CHECK does,
+src/headers/Common.h:10-20).
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import ShgParams, ShgResult
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libshg_hip.so")
+
+# every symbol include/shg.h declares
+EXPORTS = (
+    "shg_fft_size_supported", "shg_params_default", "shg_params_set_crystal",
+    "shg_refractive_index", "shg_delta_k", "shg_validate", "shg_last_error",
+    "shg_handle_create", "shg_handle_destroy", "shg_run", "shg_run_batch", "shg_adjoint",
+    "shg_cli_main", "shg_last_kernel_times", "shg_handle_set_profiling", "shg_set_plan",
+    "shg_autotune",
+)
+
+_lib = None
+
+
+class ShgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{abi.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ShgError(abi.SHG_EUNSUPPORTED,
+                       f"{LIB_PATH} not built; run __graft_entry__.build() or cushg_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER(ShgParams)
+    R = C.POINTER(ShgResult)
+    H = C.c_void_p
+    L.shg_fft_size_supported.argtypes = [C.c_int32]
+    L.shg_params_default.argtypes = [P]
+    L.shg_params_default.restype = None
+    L.shg_params_set_crystal.argtypes = [P, C.c_int32]
+    L.shg_refractive_index.argtypes = [C.c_int32, C.c_double, C.c_double]
+    L.shg_refractive_index.restype = C.c_double
+    L.shg_delta_k.argtypes = [P, C.c_double]
+    L.shg_delta_k.restype = C.c_double
+    L.shg_validate.argtypes = [P]
+    L.shg_last_error.restype = C.c_char_p
+    L.shg_handle_create.argtypes = [C.c_int32, C.POINTER(H)]
+    L.shg_handle_destroy.argtypes = [H]
+    L.shg_run.argtypes = [H, P, R]
+    L.shg_run_batch.argtypes = [H, P, C.c_int32, R]
+    L.shg_adjoint.argtypes = [H, P, C.c_int32, C.POINTER(C.c_double), R]
+    L.shg_cli_main.argtypes = [C.c_int, C.POINTER(C.c_char_p), P, C.c_char_p]
+    L.shg_last_kernel_times.argtypes = [H, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int32), C.c_int32]
+    L.shg_handle_set_profiling.argtypes = [H, C.c_int32]
+    L.shg_set_plan.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
+    L.shg_autotune.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return (lib().shg_last_error() or b"").decode()
+
+
+def _check(rc: int):
+    if rc != abi.SHG_OK:
+        raise ShgError(rc, last_error())
+
+
+def default_params() -> ShgParams:
+    """Reference defaults as filled by the library (shg_params_default)."""
+    p = ShgParams()
+    lib().shg_params_default(C.byref(p))
+    return p
+
+
+def validate(p: ShgParams) -> int:
+    return lib().shg_validate(C.byref(p))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Handle:
+    """Owns device buffers, plans and the HIP stream (shg_handle)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().shg_handle_create(device, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().shg_handle_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _prep_result(self, p: ShgParams, fields: bool, temperature: bool):
+        r = ShgResult()
+        out = {}
+        if fields:
+            out["pump"] = np.zeros((p.ny, p.nx), np.complex128)
+            out["signal"] = np.zeros((p.ny, p.nx), np.complex128)
+            r.pump_out = _dp(out["pump"].view(np.float64))
+            r.signal_out = _dp(out["signal"].view(np.float64))
+        if temperature and p.thermal:
+            nzh = p.nz_heat if p.nz_heat > 0 else p.nz
+            out["T"] = np.zeros((nzh, p.ny, p.nx), np.float64)
+            r.temperature_out = _dp(out["T"])
+        return r, out
+
+    def run(self, p: ShgParams, fields: bool = True, temperature: bool = False):
+        """shg_run: returns (ShgResult, dict of output arrays)."""
+        r, out = self._prep_result(p, fields, temperature)
+        _check(lib().shg_run(self._h, C.byref(p), C.byref(r)))
+        return r, out
+
+    def run_batch(self, ps, fields: bool = False, temperature: bool = False):
+        """shg_run_batch over a list of ShgParams; per-instance status in r.status."""
+        n = len(ps)
+        arr = (ShgParams * n)()
+        for i, p in enumerate(ps):
+            C.memmove(C.byref(arr[i]), C.byref(p), C.sizeof(ShgParams))
+        res = (ShgResult * n)()
+        outs = []
+        for i, p in enumerate(ps):
+            r, out = self._prep_result(p, fields, temperature)
+            res[i] = r
+            outs.append(out)
+        rc = lib().shg_run_batch(self._h, arr, n, res)
+        return rc, list(res), outs
+
+    def adjoint(self, p: ShgParams, checkpoint_every: int = 0):
+        """shg_adjoint: returns (ShgResult, d_eta array [waist, focus, T, poling...])."""
+        g = np.zeros(3 + p.nz, np.float64)
+        r = ShgResult()
+        _check(lib().shg_adjoint(self._h, C.byref(p), checkpoint_every, _dp(g), C.byref(r)))
+        return r, g
+
+    def set_profiling(self, on: bool):
+        _check(lib().shg_handle_set_profiling(self._h, int(on)))
+
+    def kernel_times(self) -> dict:
+        cap = 64
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        nl = (C.c_int32 * cap)()
+        n = lib().shg_last_kernel_times(self._h, names, ms, nl, cap)
+        return {names[i].decode(): (ms[i], nl[i]) for i in range(n)}
+
+    def set_plan(self, n: int, rows: int = 0, cols: int = 0):
+        _check(lib().shg_set_plan(self._h, n, rows, cols))
+
+    def autotune(self, n: int, batch: int = 1, nz: int = 64):
+        _check(lib().shg_autotune(self._h, n, batch, nz))
+
+
+def save_matrix_complex(A: np.ndarray, name: str, out_dir: str = ".") -> None:
+    """Text format of saveMatrixComplex (src/headers/Files.h:69-90):
+    <name>_r.dat / <name>_i.dat, ny rows of nx tab-terminated values."""
+    for suf, part in (("_r.dat", A.real), ("_i.dat", A.imag)):
+        with open(os.path.join(out_dir, name + suf), "w") as f:
+            for row in part.astype(np.float32):
+                f.write("".join(f"{float(v):.20g}\t" for v in row) + "\n")
+
+
+class Solver:
+    """Mirror of the reference's class Solver (src/headers/Solver.h:116-150)
+    over the C ABI; `params` holds what the reference fixes at compile time."""
+
+    def __init__(self, params: ShgParams | None = None, device: int = 0):
+        self.params = params.copy() if params is not None else default_params()
+        self.handle = Handle(device)
+
+    def run(self, Power, waist, focalpoint, Temp, T_inf, Tpeltier, save_only_last=True,
+            save_temperature=False, out_dir=None):
+        p = self.params.copy()
+        p.power_W, p.waist_um, p.focal_um = float(Power), float(waist), float(focalpoint)
+        p.T_C, p.T_inf_C, p.T_peltier_C = float(Temp), float(T_inf), float(Tpeltier)
+        r, out = self.handle.run(p, fields=True, temperature=bool(save_temperature))
+        if out_dir is not None:
+            if not save_only_last:
+                raise ShgError(abi.SHG_EUNSUPPORTED, "per-slice dumps (save_only_last=0) not provided")
+            save_matrix_complex(out["pump"], "Pump_out", out_dir)
+            save_matrix_complex(out["signal"], "Signal_out", out_dir)
+        return r, out

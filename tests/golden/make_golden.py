@@ -1,0 +1,165 @@
+"""Writes the golden fixtures under tests/golden/ (run in the build container).
+
+The reference ships no fixtures, known-answer tests or golden vectors for
+this path (its repository has no tests at all), so every fixture here comes
+from an independent source:
+  fft_vectors.npz   numpy.fft (pocketfft) outputs for seeded complex inputs
+                    of every GPU transform length, 1D and 2D.
+  heat_fv.npz       steady states of the open-top heat problem assembled
+                    independently as a finite-volume sparse matrix (face
+                    conductances k*A/d, convective faces h*A, Dirichlet bottom)
+                    and solved with scipy.sparse.linalg.spsolve.
+  sellmeier.json    n(lambda, T) and dk(T) evaluated with Python's decimal
+                    module at 40 significant digits from the published
+                    coefficients (Bruner et al. for MgO:sPPLT, as in
+                    src/headers/Crystal.h:54-70).
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+from decimal import Decimal, getcontext
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def fft_vectors():
+    rng = np.random.default_rng(20260101)
+    out = {}
+    for n in (64, 128, 192, 256, 384, 512):
+        x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))).astype(np.complex128)
+        out[f"x{n}"] = x
+        out[f"X{n}"] = np.fft.fft(x, axis=-1)
+        out[f"I{n}"] = np.fft.ifft(x, axis=-1) * n  # unnormalised inverse
+    for (ny, nx) in ((64, 64), (128, 192)):
+        x = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(np.complex128)
+        out[f"x2d_{ny}_{nx}"] = x
+        out[f"X2d_{ny}_{nx}"] = np.fft.fft2(x)
+    np.savez_compressed(os.path.join(HERE, "fft_vectors.npz"), **out)
+
+
+def fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Q):
+    """Finite-volume steady heat equation on the node grid of the reference
+    (dx = lx/(nx-1) ...), each node owning the cell [x-dx/2, x+dx/2] clipped
+    to the crystal; faces x=0, x=lx, y=ly, z=0, z=lz convective (h), the
+    y = 0 plane held at T_pel, Q applied to interior nodes only."""
+    dx, dy, dz = lx / (nx - 1), ly / (ny - 1), lz / (nz - 1)
+
+    def width(i, n, d):
+        return d / 2 if i in (0, n - 1) else d
+
+    idx = -np.ones((nz, ny, nx), dtype=np.int64)
+    cnt = 0
+    for z in range(nz):
+        for y in range(1, ny):
+            for x in range(nx):
+                idx[z, y, x] = cnt
+                cnt += 1
+    rows, cols, vals = [], [], []
+    b = np.zeros(cnt)
+    for z in range(nz):
+        for y in range(1, ny):
+            for x in range(nx):
+                i = idx[z, y, x]
+                wx, wy, wz = width(x, nx, dx), width(y, ny, dy), width(z, nz, dz)
+                diag = 0.0
+                # conductive links
+                for (xx, yy, zz, area, d) in (
+                    (x - 1, y, z, wy * wz, dx), (x + 1, y, z, wy * wz, dx),
+                    (x, y - 1, z, wx * wz, dy), (x, y + 1, z, wx * wz, dy),
+                    (x, y, z - 1, wx * wy, dz), (x, y, z + 1, wx * wy, dz),
+                ):
+                    if not (0 <= xx < nx and 0 <= yy < ny and 0 <= zz < nz):
+                        continue
+                    g = k * area / d
+                    diag += g
+                    if yy == 0:
+                        b[i] += g * T_pel
+                    else:
+                        rows.append(i)
+                        cols.append(idx[zz, yy, xx])
+                        vals.append(-g)
+                # convective faces
+                conv = 0.0
+                if x == 0 or x == nx - 1:
+                    conv += h * wy * wz
+                if y == ny - 1:
+                    conv += h * wx * wz
+                if z == 0 or z == nz - 1:
+                    conv += h * wx * wy
+                diag += conv
+                b[i] += conv * T_inf
+                interior = 0 < x < nx - 1 and 0 < y < ny - 1 and 0 < z < nz - 1
+                if interior:
+                    b[i] += Q[z, y, x] * wx * wy * wz
+                rows.append(i)
+                cols.append(i)
+                vals.append(diag)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(cnt, cnt))
+    t = spla.spsolve(A.tocsc(), b)
+    T = np.full((nz, ny, nx), T_pel)
+    for z in range(nz):
+        for y in range(1, ny):
+            for x in range(nx):
+                T[z, y, x] = t[idx[z, y, x]]
+    return T
+
+
+def heat_fv():
+    rng = np.random.default_rng(7)
+    cases = {}
+    specs = [
+        # nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Qscale
+        (8, 8, 6, 800.0, 800.0, 30000.0, 8.4e-6, 10e-12, 25.0, 40.0, 1e-9),
+        (6, 5, 4, 2000.0, 1000.0, 30000.0, 8.4e-6, 10e-12, 25.0, 55.66, 5e-9),
+        (10, 7, 5, 800.0, 600.0, 1000.0, 4.6e-6, 5e-8, 20.0, 35.0, 2e-10),
+    ]
+    for ci, (nx, ny, nz, lx, ly, lz, k, h, Ti, Tp, qs) in enumerate(specs):
+        Q = rng.random((nz, ny, nx)) * qs
+        T = fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, Ti, Tp, Q)
+        cases[f"spec{ci}"] = np.array([nx, ny, nz, lx, ly, lz, k, h, Ti, Tp], dtype=np.float64)
+        cases[f"Q{ci}"] = Q
+        cases[f"T{ci}"] = T
+    np.savez_compressed(os.path.join(HERE, "heat_fv.npz"), **cases)
+
+
+def sellmeier():
+    getcontext().prec = 40
+    D = Decimal
+
+    def n_spplt(L, T):
+        A, B, Cc, Dd = D("4.502483"), D("0.007294"), D("0.185087"), D("-0.02357")
+        E, F, G, H = D("0.073423"), D("0.199595"), D("0.001"), D("7.99724")
+        tk = D(T) + D("273.15")
+        b = D("3.483933e-8") * tk * tk
+        c = D("1.607839e-8") * tk * tk
+        L = D(L)
+        L2 = L * L
+        return (A + (B + b) / (L2 - (Cc + c) ** 2) + E / (L2 - F * F) + G / (L2 - H * H) + Dd * L2).sqrt()
+
+    pi = D("3.141592653589793238462643383279502884197")
+
+    def dk(T, Lambda="7.97", a="2.2e-6", b="-5.9e-9"):
+        lp, ls = D("1.064"), D("0.532")
+        d = D(T) - D(25)
+        return 2 * pi * (2 * n_spplt(lp, T) / lp - n_spplt(ls, T) / ls
+                         + 1 / (D(Lambda) * (1 + D(a) * d + D(b) * d * d)))
+
+    rows = []
+    for T in ("25", "40", "55.66", "60"):
+        rows.append({"T": float(T), "n_p": float(n_spplt("1.064", T)), "n_s": float(n_spplt("0.532", T)),
+                     "dk": float(dk(T))})
+    with open(os.path.join(HERE, "sellmeier.json"), "w") as f:
+        json.dump({"crystal": "MgO:sPPLT", "Lambda_um": 7.97, "rows": rows}, f, indent=1)
+
+
+if __name__ == "__main__":
+    fft_vectors()
+    heat_fv()
+    sellmeier()
+    print("fixtures written to", HERE)

@@ -1,0 +1,318 @@
+"""Pins the FP64 oracle (CPU, no GPU needed).
+
+Each check is chosen so that a plausible restatement error (dropped term,
+wrong sign or index, transposed operand) fails it:
+  - FFT against numpy golden vectors and a long-double naive DFT;
+  - n(lambda,T) and dk(T) against a 40-digit decimal evaluation and the
+    phase-matching temperature;
+  - pump + propagator + FFT normalisation against the analytic Gaussian
+    beam after free-space propagation;
+  - the RK4 schemes against the closed-form SHG solution (sech/tanh);
+  - power conservation (Manley-Rowe) of the full Strang march;
+  - the Jacobi sweep and the direct heat solver against an independently
+    assembled finite-volume system solved by scipy;
+  - the THERMAL march with a uniform temperature against the non-THERMAL
+    march (pins the running dk integral and the heat-slice mapping);
+  - the reference Jacobi loop against the direct solver.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cushg_amd import abi, configs
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+C_LIGHT = 299792458.0 * 1e6 / 1e12
+EPS0 = 8.8541878128e-12 * 1e12 / 1e6
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.fixture(scope="module")
+def fftgold():
+    return np.load(os.path.join(GOLD, "fft_vectors.npz"))
+
+
+@pytest.mark.parametrize("n", [64, 128, 192, 256, 384, 512])
+def test_fft1d_golden(fftgold, n):
+    x = fftgold[f"x{n}"]
+    X = O.fft1d(x)
+    Xi = O.fft1d(x, inverse=True)
+    assert np.abs(X - fftgold[f"X{n}"]).max() / np.abs(fftgold[f"X{n}"]).max() < 1e-14
+    assert np.abs(Xi - fftgold[f"I{n}"]).max() / np.abs(fftgold[f"I{n}"]).max() < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (128, 192)])
+def test_fft2d_golden(fftgold, shape):
+    ny, nx = shape
+    x = fftgold[f"x2d_{ny}_{nx}"]
+    X = O.fft2d(x)
+    ref = fftgold[f"X2d_{ny}_{nx}"]
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-14
+
+
+@pytest.mark.parametrize("n", [7, 12, 30, 48])
+def test_fft_vs_naive_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for inv in (False, True):
+        a = O.fft1d(x, inverse=inv)
+        b = O.dft_naive(x, inverse=inv)
+        assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+
+
+def test_fft_roundtrip_parseval():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((64, 128)) + 1j * rng.standard_normal((64, 128))
+    X = O.fft2d(x)
+    assert abs(np.sum(np.abs(X) ** 2) / x.size - np.sum(np.abs(x) ** 2)) < 1e-12 * np.sum(np.abs(x) ** 2)
+    y = O.fft2d(X, inverse=True) / x.size
+    assert np.abs(y - x).max() < 1e-14
+
+
+# ---------------------------------------------------------------- material
+
+def test_sellmeier_and_dk_decimal():
+    g = json.load(open(os.path.join(GOLD, "sellmeier.json")))
+    p = abi.default_params()
+    p.Lambda_um = g["Lambda_um"]
+    for row in g["rows"]:
+        T = row["T"]
+        assert O.refractive_index(0, 1.064, T) == pytest.approx(row["n_p"], rel=1e-15)
+        assert O.refractive_index(0, 0.532, T) == pytest.approx(row["n_s"], rel=1e-15)
+        assert O.delta_k(p, T) == pytest.approx(row["dk"], rel=1e-11, abs=1e-16)
+
+
+def test_dk_zero_near_56p92C():
+    """dk vanishes at 56.9158 C for Lambda = 7.97 um (plane-wave phase matching)."""
+    p = abi.default_params()
+    lo, hi = 50.0, 60.0
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if O.delta_k(p, mid) > 0:
+            lo = mid
+        else:
+            hi = mid
+    assert abs(lo - 56.9158) < 1e-3
+
+
+def test_qpm_period_helper_phase_matches():
+    p = configs.config2()
+    assert abs(O.delta_k(p, 40.0)) < 1e-14
+
+
+def test_tridiag_eigen_vs_numpy():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 5, 40, 101):
+        d = rng.standard_normal(n)
+        o = rng.standard_normal(max(n - 1, 0))
+        lam, V = O.tridiag_eigen(d, o if n > 1 else np.zeros(1))
+        A = np.diag(d) + (np.diag(o, 1) + np.diag(o, -1) if n > 1 else 0)
+        assert np.allclose(np.sort(lam), np.linalg.eigvalsh(A), atol=1e-12)
+        assert np.abs(A @ V - V * lam).max() < 1e-12
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+
+
+# ---------------------------------------------------------------- optics
+
+def _gauss(p, Tc, zrel):
+    """Analytic paraxial Gaussian beam, focus at focal_um, observed zrel after
+    the input plane; same amplitude normalisation as the pump init."""
+    n = O.refractive_index(p.crystal, p.lambda_p_um, Tc)
+    w2 = p.waist_um ** 2
+    zR = math.pi * n * w2 / p.lambda_p_um
+    Ap0 = math.sqrt(4 * p.power_W / (EPS0 * C_LIGHT * math.pi * n * w2))
+    M = 1 + 1j * (zrel - p.focal_um) / zR
+    dx, dy = p.lx_um / (p.nx - 1), p.ly_um / (p.ny - 1)
+    x = (np.arange(p.nx) - p.nx / 2) * dx
+    y = (np.arange(p.ny) - p.ny / 2) * dy
+    r2 = x[None, :] ** 2 + y[:, None] ** 2
+    return (Ap0 / M) * np.exp(-r2 / (w2 * M))
+
+
+def test_pump_init_closed_form_and_power():
+    p = configs.config1()
+    p.nx = p.ny = 128
+    A = O.pump_init(p, p.T_C)
+    assert np.abs(A - _gauss(p, p.T_C, 0.0)).max() < 1e-12 * np.abs(A).max()
+    n = O.refractive_index(0, 1.064, p.T_C)
+    assert O.power(p, A, n) == pytest.approx(p.power_W, rel=1e-6)
+
+
+def test_free_space_gaussian_matches_analytic():
+    """No coupling, no loss: the march is pure diffraction; the saved output
+    (after nz-1 slices = the crystal length) is the analytic beam there."""
+    p = configs.config1()
+    p.nx = p.ny = 128
+    p.deff_um_per_V = 0.0
+    p.nz = 30
+    r, o = O.run(p)
+    ref = _gauss(p, p.T_C, p.lz_um)
+    assert np.abs(o["pump"] - ref).max() < 1e-8 * np.abs(ref).max()
+    assert np.abs(o["signal"]).max() == 0.0
+    # diffraction is unitary: power kept to round-off
+    assert r.P_pump_out_W == pytest.approx(r.P_pump_in_W, rel=1e-12)
+
+
+def _plane_wave_run(p, steps, P0):
+    P = np.full((p.ny, p.nx), P0, np.complex128)
+    S = np.zeros_like(P)
+    for s in range(steps):
+        P, S = O.ssfm_step(p, s, P, S)
+    return P[0, 0], S[0, 0]
+
+
+def _tanh_closed_form(p, P0, z, kmul):
+    dQ = 2 * p.deff_um_per_V / math.pi
+    n_p = O.refractive_index(0, p.lambda_p_um, p.T_C)
+    n_s = O.refractive_index(0, 0.5 * p.lambda_p_um, p.T_C)
+    kp = kmul * math.pi * dQ / (n_p * p.lambda_p_um)
+    ks = kmul * 0.5 * math.pi * dQ / (n_s * 0.5 * p.lambda_p_um)
+    G = math.sqrt(kp * ks) * P0
+    return P0 / math.cosh(G * z), 1j * math.sqrt(ks / kp) * P0 * math.tanh(G * z)
+
+
+def _pw_params(nz):
+    p = configs.config1()
+    p.alpha_p = p.alpha_s = p.beta_s = 0.0
+    p.nx = p.ny = 8
+    p.nz = nz
+    p.Lambda_um = abi.qpm_period(p, p.T_C)
+    return p
+
+
+def test_strang_rk4_matches_tanh():
+    """Plane wave at dk = 0: textbook SHG, P = P0 sech(Gz), S = i sqrt(ks/kp) P0 tanh(Gz)."""
+    P0 = 1.5
+    p = _pw_params(41)
+    p.scheme = abi.SCHEME_STRANG
+    steps = p.nz - 1
+    dz = p.lz_um / (p.nz - 1)
+    P, S = _plane_wave_run(p, steps, P0)
+    Pe, Se = _tanh_closed_form(p, P0, steps * dz, 2.0)
+    assert abs(S - Se) / abs(Se) < 1e-9
+    assert abs(P - Pe) / abs(Pe) < 1e-9
+
+
+def test_reference_scheme_first_order_to_tanh():
+    """The reference RK4 (stage inputs without dz, full-dz half slices) is a
+    first-order integrator of the same ODE with the textbook coupling:
+    halving dz halves the error."""
+    P0 = 1.5
+    errs = []
+    for nz in (21, 41, 81):
+        p = _pw_params(nz)
+        steps = p.nz - 1
+        P, S = _plane_wave_run(p, steps, P0)
+        _, Se = _tanh_closed_form(p, P0, p.lz_um, 2.0)
+        errs.append(abs(S - Se) / abs(Se))
+    assert errs[2] < 2e-2
+    assert 1.7 < errs[0] / errs[1] < 2.3
+    assert 1.7 < errs[1] / errs[2] < 2.3
+
+
+def test_strang_march_conserves_power():
+    """Manley-Rowe: with alpha = beta = 0, n_p|P|^2 + n_s|S|^2 is invariant."""
+    p = configs.config1()
+    p.scheme = abi.SCHEME_STRANG
+    p.power_W = 50.0
+    p.nz = 60
+    r, _ = O.run(p)
+    assert r.P_sh_W > 1e-3 * p.power_W
+    assert (r.P_pump_out_W + r.P_sh_W) == pytest.approx(r.P_pump_in_W, rel=1e-7)
+
+
+# ---------------------------------------------------------------- heat
+
+@pytest.fixture(scope="module")
+def heatgold():
+    return np.load(os.path.join(GOLD, "heat_fv.npz"))
+
+
+def _heat_params(spec):
+    nx, ny, nz, lx, ly, lz, k, h, Ti, Tp = spec
+    p = abi.default_params()
+    p.nx, p.ny = int(nx), int(ny)
+    p.nz = int(nz)
+    p.lx_um, p.ly_um, p.lz_um = lx, ly, lz
+    p.k_th, p.h_conv, p.T_inf_C, p.T_peltier_C = k, h, Ti, Tp
+    return p, int(nz)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_direct_heat_solve_vs_scipy_fv(heatgold, case):
+    p, nzh = _heat_params(heatgold[f"spec{case}"])
+    T = O.heat_solve_direct(p, nzh, heatgold[f"Q{case}"])
+    ref = heatgold[f"T{case}"]
+    assert np.abs(T - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_jacobi_sweep_fixed_point_is_fv_solution(heatgold, case):
+    """Every boundary formula of the reference sweep (faces, edges, vertices)
+    leaves the finite-volume steady state unchanged."""
+    p, nzh = _heat_params(heatgold[f"spec{case}"])
+    ref = heatgold[f"T{case}"]
+    Tf = O.jacobi_sweep(p, nzh, ref, heatgold[f"Q{case}"])
+    assert np.abs(Tf - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_jacobi_iteration_converges_to_direct(heatgold):
+    p, nzh = _heat_params(heatgold["spec1"])
+    Q = heatgold["Q1"]
+    T = np.full_like(Q, p.T_peltier_C)
+    for _ in range(20000):
+        Tn = O.jacobi_sweep(p, nzh, T, Q)
+        if np.abs(Tn - T).max() < 1e-13:
+            break
+        T = Tn
+    assert np.abs(T - heatgold["T1"]).max() < 1e-9
+
+
+def test_thermal_uniform_equals_plain():
+    """T_inf = T_oven = T and no absorption: T stays uniform, the running dk
+    integral reduces to z*dk and the THERMAL march equals the plain one."""
+    p = configs.config1()
+    p.nx = p.ny = 32
+    p.nz = 24
+    p.T_inf_C = p.T_peltier_C = p.T_C
+    p.alpha_p = p.alpha_s = p.beta_s = 0.0
+    r0, o0 = O.run(p)
+    for nzh, solver in ((24, abi.HEAT_DIRECT), (7, abi.HEAT_DIRECT), (24, abi.HEAT_JACOBI_REFERENCE)):
+        q = p.copy()
+        q.thermal = 1
+        q.nz_heat = nzh
+        q.heat_solver = solver
+        r1, o1 = O.run(q)
+        assert r1.eta == pytest.approx(r0.eta, rel=1e-12)
+        assert np.abs(o1["signal"] - o0["signal"]).max() < 1e-12 * np.abs(o0["signal"]).max()
+        assert r1.outer_iters == 2  # pass 1 solves, pass 2 converges in one sweep
+
+
+def test_thermal_jacobi_reference_loop_vs_direct():
+    """With a tight tolerance the reference Jacobi loop reaches the direct
+    solver's steady state, so the thermo-optic results agree."""
+    p = configs.config2()
+    p.nx = p.ny = 16
+    p.nz = 12
+    p.nz_heat = 6
+    p.lx_um = p.ly_um = 200.0
+    p.lz_um = 2000.0
+    p.power_W = 50.0
+    p.waist_um = 20.0
+    p.alpha_p, p.alpha_s = 2e-4, 2e-3
+    p.focal_um = 1000.0
+    p.jacobi_tol = 1e-11
+    p.check_every = 50
+    rd, od = O.run(p, temperature=True)
+    q = p.copy()
+    q.heat_solver = abi.HEAT_JACOBI_REFERENCE
+    rj, oj = O.run(q, temperature=True)
+    assert rd.peak_dT_K > 1e-3
+    assert np.abs(oj["T"] - od["T"]).max() < 1e-8
+    assert rj.eta == pytest.approx(rd.eta, rel=1e-9)
+    assert rj.jacobi_sweeps > rd.jacobi_sweeps
