@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "runtime.hpp"
+#include <rocblas/rocblas.h>
 
 namespace shg {
 const char* last_error();
@@ -199,6 +200,7 @@ int shg_handle_destroy(shg_handle* h) {
     if (h->arena.base) (void)hipFree(h->arena.base);
     if (h->d_scalars) (void)hipFree(h->d_scalars);
     if (h->d_partials) (void)hipFree(h->d_partials);
+    if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
     (void)hipStreamDestroy(h->stream);
     delete h;
     return SHG_OK;

@@ -1,0 +1,465 @@
+// heat.cpp -- THERMAL host orchestration: the steady heat solve on the GPU
+// and the optics inputs derived from T.
+//
+// Two solvers reach the same discrete steady state, the fixed point of the
+// reference's Jacobi map (src/headers/Tfield.h:93-201):
+//   JACOBI_REFERENCE  the reference loop itself (src/headers/Solver.h:332-356):
+//                     FP64 sweeps, ||T_f - T_i||_2 tested every check_every.
+//   DIRECT            defect correction  T <- T + M^-1 r,  r = A_ii (F(T) - T)
+//                     the FP64 residual of that same map, M^-1 the exact
+//                     inverse of its tensor-product finite-volume form by
+//                     fast diagonalisation (three axis transforms = plain
+//                     GEMMs, FP32 in the mixed form, FP64 in the plain form),
+//                     repeated until max|update| < heat_tol_K.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "heat.hpp"
+#include "heat_kernels.hpp"
+#include "runtime.hpp"
+
+namespace shg {
+
+HeatCoef heat_coef(const shg_params* p, int nzh) {
+    HeatCoef c{};
+    c.nx = p->nx;
+    c.ny = p->ny;
+    c.nz = nzh;
+    const double dx = p->lx_um / (p->nx - 1), dy = p->ly_um / (p->ny - 1), dz = p->lz_um / (nzh - 1);
+    const double kt = p->k_th, hc = p->h_conv;
+    c.kth = kt;
+    c.Tinf = p->T_inf_C;
+    c.idx2 = 1.0 / (dx * dx);
+    c.idy2 = 1.0 / (dy * dy);
+    c.idz2 = 1.0 / (dz * dz);
+    c.L2 = 1.0 / (2 / (dx * dx) + 2 / (dy * dy) + 2 / (dz * dz));
+    c.Bix = hc * dx / kt;
+    c.Biy = hc * dy / kt;
+    c.Biz = hc * dz / kt;
+    c.ax = dx * dx / (2 * dy * dy);
+    c.ay = dy * dy / (2 * dx * dx);
+    c.az = dz * dz / (2 * dx * dx);
+    c.bx = dx * dx / (2 * dz * dz);
+    c.by = dy * dy / (2 * dz * dz);
+    c.bz = dz * dz / (2 * dy * dy);
+    c.av = dz / (2 * dx);
+    c.bv = dx / (2 * dz);
+    c.cv = dx * dz / (4 * dy * dy);
+    c.V = 1 / (c.av + c.bv + 2 * c.cv + 0.5 * (c.Bix + c.Biz));
+    c.ah = dz / (2 * dy);
+    c.bh = dy / (2 * dz);
+    c.ch = dy * dz / (4 * dx * dx);
+    c.H = 1 / (c.ah + c.bh + 2 * c.ch + 0.5 * (c.Biy + c.Biz));
+    c.ad = dx / (2 * dy);
+    c.bd = dy / (2 * dx);
+    c.cd = dx * dy / (4 * dz * dz);
+    c.D = 1 / (c.ad + c.bd + 2 * c.cd + 0.5 * (c.Bix + c.Biy));
+    c.AV = dy * dz / dx;
+    c.BV = dx * dz / dy;
+    c.CV = dx * dy / dz;
+    c.BiD = c.Bix * dy + c.Biy * dz + c.Biz * dx;
+    c.GV = 1 / (c.AV + c.BV + c.CV + c.BiD);
+    return c;
+}
+
+// Eigen-decomposition of a symmetric tridiagonal matrix (diagonal d, first
+// off-diagonal e) by implicit QL sweeps with a Wilkinson shift
+// (Golub & Van Loan, Matrix Computations, sec. 8.3).  On return d holds the
+// eigenvalues and z[j*n + k] component j of eigenvector k.
+static bool sym_tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vector<double>& z) {
+    z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) z[(size_t)i * n + i] = 1.0;
+    e.resize(n + 1, 0.0);
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        for (int sweep = 0;; sweep++) {
+            int m = l;
+            while (m < n - 1) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) + dd == dd) break;
+                m++;
+            }
+            if (m == l) break;
+            if (sweep > 300) return false;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool early = false;
+            for (int i = m - 1; i >= l; i--) {
+                double f = s * e[i];
+                const double b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    early = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < n; k++) {
+                    double* zk = &z[(size_t)k * n];
+                    f = zk[i + 1];
+                    zk[i + 1] = s * zk[i] + c * f;
+                    zk[i] = c * zk[i] - s * f;
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    return true;
+}
+
+// One axis of the finite-volume form (values divided by k_th): node widths
+// w, stiffness diagonal k (1/h per neighbour link, + h_conv/k_th at Robin
+// ends), eigenpairs of the symmetric W^-1/2 K W^-1/2.
+struct AxisHost {
+    int n = 0;
+    std::vector<double> k, w, lam, q;
+};
+static bool axis_build(int n, double h, bool robin_lo, bool robin_hi, bool dirichlet_lo, double hk, AxisHost& a) {
+    a.n = n;
+    a.k.resize(n);
+    a.w.resize(n);
+    std::vector<double> diag(n), off(std::max(n - 1, 1), 0.0);
+    for (int i = 0; i < n; i++) {
+        const bool lo_end = i == 0 && !dirichlet_lo, hi_end = i == n - 1;
+        a.w[i] = (lo_end || hi_end) ? 0.5 * h : h;
+        double kd = 0;
+        if (i > 0 || dirichlet_lo) kd += 1.0 / h;
+        if (i < n - 1) kd += 1.0 / h;
+        if (i == 0 && robin_lo) kd += hk;
+        if (i == n - 1 && robin_hi) kd += hk;
+        a.k[i] = kd;
+        diag[i] = kd / a.w[i];
+    }
+    for (int i = 0; i + 1 < n; i++) off[i] = -1.0 / h / std::sqrt(a.w[i] * a.w[i + 1]);
+    a.lam = diag;
+    return sym_tridiag_eig(n, a.lam, off, a.q);
+}
+
+// Device-side tables of the direct solver (carved from the run arena).
+struct DirectTables {
+    HeatAxes ax{};
+    const void* q[3] = {nullptr, nullptr, nullptr};  // eigenvector matrices (R precision)
+    void* u = nullptr;
+    void* v = nullptr;
+    bool fp64 = false;
+};
+
+static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t heat_bytes(const shg_params* p) {
+    const size_t nzh = p->nz_heat > 0 ? p->nz_heat : p->nz;
+    const size_t n3 = (size_t)p->nx * p->ny * nzh;
+    const size_t nu = (size_t)p->nx * (p->ny - 1) * nzh;
+    const size_t rs = p->precision == SHG_PREC_FP64_PLAIN ? 8 : 4;
+    size_t b = 4 * al256(8 * n3) + 2 * al256(4 * n3);  // T, Tn, dk, dkbase; T32, Q
+    b += 2 * al256(rs * nu);                           // u, v
+    const size_t nb = (size_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * nzh;
+    b += al256(8 * nb) + al256(8 * 8);                 // partials, slots
+    b += al256(8 * (size_t)p->nz);                     // poling
+    const size_t n[3] = {(size_t)p->nx, (size_t)p->ny - 1, nzh};
+    for (int i = 0; i < 3; i++) b += 3 * al256(8 * n[i]) + al256(rs * n[i] * n[i]);
+    b += al256(sizeof(DirectTables)) + 4096;
+    return b;
+}
+
+static rocblas_handle blas(shg_handle* h) {
+    if (!h->blas) {
+        rocblas_handle bh = nullptr;
+        if (rocblas_create_handle(&bh) != rocblas_status_success) return nullptr;
+        rocblas_set_stream(bh, h->stream);
+        h->blas = bh;
+    }
+    return (rocblas_handle)h->blas;
+}
+
+int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
+    hs.nx = p->nx;
+    hs.ny = p->ny;
+    hs.nzh = p->nz_heat > 0 ? p->nz_heat : p->nz;
+    hs.dx = p->lx_um / (p->nx - 1);
+    hs.dy = p->ly_um / (p->ny - 1);
+    hs.dz = p->lz_um / (hs.nzh - 1);
+    const size_t n3 = (size_t)hs.nx * hs.ny * hs.nzh;
+    const size_t nu = (size_t)hs.nx * (hs.ny - 1) * hs.nzh;
+    const bool f64 = p->precision == SHG_PREC_FP64_PLAIN;
+    Arena& A = h->arena;
+    hs.T = A.carve<double>(n3);
+    hs.Tn = A.carve<double>(n3);
+    hs.dk = A.carve<double>(n3);
+    hs.dkbase = A.carve<double>(n3);
+    hs.T32 = A.carve<float>(n3);
+    hs.Q = A.carve<float>(n3);
+    const HeatCoef c = heat_coef(p, hs.nzh);
+    hs.nblocks = heat_grid_blocks(c);
+    hs.part = A.carve<double>(hs.nblocks);
+    hs.red = A.carve<double>(8);
+    hs.pol = nullptr;
+    if (p->poling_phase) {
+        hs.pol = A.carve<double>(p->nz);
+        if (hs.pol)
+            SHG_HIP(hipMemcpyAsync(hs.pol, p->poling_phase, sizeof(double) * p->nz, hipMemcpyHostToDevice, h->stream));
+    }
+    DirectTables* dt = new DirectTables();
+    hs.solver = dt;
+    dt->fp64 = f64;
+    dt->u = f64 ? (void*)A.carve<double>(nu) : (void*)A.carve<float>(nu);
+    dt->v = f64 ? (void*)A.carve<double>(nu) : (void*)A.carve<float>(nu);
+    if (!hs.T || !hs.Q || !hs.red || !dt->u || !dt->v) {
+        set_error("arena exhausted (heat buffers)");
+        return SHG_ENOMEM;
+    }
+    // axes: x (Robin both ends), y' = y-1 (Dirichlet below, Robin top), z (Robin both)
+    const double hk = p->h_conv / p->k_th;
+    AxisHost axes[3];
+    const bool ok = axis_build(hs.nx, hs.dx, true, true, false, hk, axes[0]) &&
+                    axis_build(hs.ny - 1, hs.dy, false, true, true, hk, axes[1]) &&
+                    axis_build(hs.nzh, hs.dz, true, true, false, hk, axes[2]);
+    if (!ok) {
+        set_error("heat: tridiagonal eigen-solver did not converge");
+        return SHG_ENOCONV;
+    }
+    const double** kp[3] = {&dt->ax.kx, &dt->ax.ky, &dt->ax.kz};
+    const double** wp[3] = {&dt->ax.wx, &dt->ax.wy, &dt->ax.wz};
+    const double** lp[3] = {&dt->ax.lx, &dt->ax.ly, &dt->ax.lz};
+    for (int i = 0; i < 3; i++) {
+        const int n = axes[i].n;
+        double* dk_ = A.carve<double>(n);
+        double* dw = A.carve<double>(n);
+        double* dl = A.carve<double>(n);
+        if (!dk_ || !dw || !dl) {
+            set_error("arena exhausted (heat tables)");
+            return SHG_ENOMEM;
+        }
+        SHG_HIP(hipMemcpy(dk_, axes[i].k.data(), 8 * n, hipMemcpyHostToDevice));
+        SHG_HIP(hipMemcpy(dw, axes[i].w.data(), 8 * n, hipMemcpyHostToDevice));
+        SHG_HIP(hipMemcpy(dl, axes[i].lam.data(), 8 * n, hipMemcpyHostToDevice));
+        *kp[i] = dk_;
+        *wp[i] = dw;
+        *lp[i] = dl;
+        if (f64) {
+            double* q = A.carve<double>((size_t)n * n);
+            if (!q) return SHG_ENOMEM;
+            SHG_HIP(hipMemcpy(q, axes[i].q.data(), 8 * (size_t)n * n, hipMemcpyHostToDevice));
+            dt->q[i] = q;
+        } else {
+            std::vector<float> qf(axes[i].q.begin(), axes[i].q.end());
+            float* q = A.carve<float>((size_t)n * n);
+            if (!q) return SHG_ENOMEM;
+            SHG_HIP(hipMemcpy(q, qf.data(), 4 * (size_t)n * n, hipMemcpyHostToDevice));
+            dt->q[i] = q;
+        }
+    }
+    return SHG_OK;
+}
+
+void heat_release(HeatState& hs) {
+    delete static_cast<DirectTables*>(hs.solver);
+    hs.solver = nullptr;
+}
+
+int heat_init(shg_handle* h, const shg_params* p, HeatState& hs) {
+    ProfScope ps(h, "heat_init");
+    SHG_HIP(launch_heat_init(h->stream, hs.T, hs.Q, hs.nx, hs.ny, hs.nzh, p->T_C, p->T_peltier_C));
+    return SHG_OK;
+}
+
+static int read_scalar(shg_handle* h, const double* d, double* out) {
+    SHG_HIP(hipMemcpyAsync(out, d, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SHG_HIP(hipStreamSynchronize(h->stream));
+    return SHG_OK;
+}
+
+// out[..i..] = sum_j M[j][i] in[..j..] along one axis of u[nz][nyu][nx]
+// (M = Q forward, Q^T inverse; Q row-major, Q[j*n+i] = component j of vector i)
+template <typename R>
+static int axis_gemm(shg_handle* h, int axis, bool fwd, const R* q, const R* in, R* out, int nx, int nyu, int nz) {
+    rocblas_handle bh = blas(h);
+    if (!bh) {
+        set_error("rocBLAS handle creation failed");
+        return SHG_EDEVICE;
+    }
+    const R one = 1, zero = 0;
+    rocblas_status st;
+    const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
+    if constexpr (sizeof(R) == 4) {
+        if (axis == 0)
+            st = rocblas_sgemm(bh, fwd ? N_ : T_, N_, nx, nyu * nz, nx, &one, q, nx, in, nx, &zero, out, nx);
+        else if (axis == 1)
+            st = rocblas_sgemm_strided_batched(bh, N_, fwd ? T_ : N_, nx, nyu, nyu, &one, in, nx, (rocblas_stride)nx * nyu,
+                                               q, nyu, 0, &zero, out, nx, (rocblas_stride)nx * nyu, nz);
+        else
+            st = rocblas_sgemm(bh, N_, fwd ? T_ : N_, nx * nyu, nz, nz, &one, in, nx * nyu, q, nz, &zero, out, nx * nyu);
+    } else {
+        if (axis == 0)
+            st = rocblas_dgemm(bh, fwd ? N_ : T_, N_, nx, nyu * nz, nx, &one, q, nx, in, nx, &zero, out, nx);
+        else if (axis == 1)
+            st = rocblas_dgemm_strided_batched(bh, N_, fwd ? T_ : N_, nx, nyu, nyu, &one, in, nx, (rocblas_stride)nx * nyu,
+                                               q, nyu, 0, &zero, out, nx, (rocblas_stride)nx * nyu, nz);
+        else
+            st = rocblas_dgemm(bh, N_, fwd ? T_ : N_, nx * nyu, nz, nz, &one, in, nx * nyu, q, nz, &zero, out, nx * nyu);
+    }
+    if (st != rocblas_status_success) {
+        set_error("rocBLAS gemm failed (%d) on axis %d", (int)st, axis);
+        return SHG_EDEVICE;
+    }
+    return SHG_OK;
+}
+
+// one defect-correction step: u = W^-1/2 r; T += W^-1/2 S^-1 u
+template <typename R>
+static int direct_step(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* r0_sq,
+                       double* max_corr) {
+    const int nx = hs.nx, nyu = hs.ny - 1, nz = hs.nzh;
+    R* u = static_cast<R*>(dt.u);
+    R* v = static_cast<R*>(dt.v);
+    const R* qx = static_cast<const R*>(dt.q[0]);
+    const R* qy = static_cast<const R*>(dt.q[1]);
+    const R* qz = static_cast<const R*>(dt.q[2]);
+    {
+        ProfScope ps(h, "heat_resid");
+        SHG_HIP(launch_heat_resid<R>(h->stream, c, dt.ax, hs.T, hs.Q, u, hs.part));
+        SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, false, hs.red + 0));
+    }
+    if (r0_sq) {
+        if (int rc = read_scalar(h, hs.red + 0, r0_sq)) return rc;
+        if (!max_corr) return SHG_OK;
+    }
+    {
+        ProfScope ps(h, "heat_fd_gemm");
+        if (int rc = axis_gemm<R>(h, 0, true, qx, u, v, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 1, true, qy, v, u, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 2, true, qz, u, v, nx, nyu, nz)) return rc;
+        SHG_HIP(launch_fd_divide<R>(h->stream, v, dt.ax, nx, nyu, nz));
+        if (int rc = axis_gemm<R>(h, 2, false, qz, v, u, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 1, false, qy, u, v, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 0, false, qx, v, u, nx, nyu, nz)) return rc;
+    }
+    {
+        ProfScope ps(h, "heat_update");
+        SHG_HIP(launch_fd_update<R>(h->stream, c, dt.ax, u, hs.T, hs.part));
+        SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, true, hs.red + 1));
+    }
+    return read_scalar(h, hs.red + 1, max_corr);
+}
+
+int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo& info) {
+    const HeatCoef c = heat_coef(p, hs.nzh);
+    info = HeatSolveInfo{};
+    if (p->heat_solver == SHG_HEAT_JACOBI_REFERENCE) {
+        // src/headers/Solver.h:332-356 (norm ||T_f - T_i||_2 instead of
+        // sqrt(sum)/TSIZE against 5e-4/TSIZE: the same test)
+        const long long chk = p->check_every > 0 ? p->check_every : 5000;
+        double red = INFINITY, aux = NAN;
+        long long counter = 0;
+        while (red >= p->jacobi_tol) {
+            const bool check = counter % chk == 0;
+            {
+                ProfScope ps(h, "heat_jacobi_sweep");
+                SHG_HIP(launch_heat_sweep(h->stream, c, hs.T, hs.Tn, hs.Q, hs.part, check));
+            }
+            if (check) {
+                SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, false, hs.red + 0));
+                double s2 = 0;
+                if (int rc = read_scalar(h, hs.red + 0, &s2)) return rc;
+                if (counter > 0) aux = red;
+                red = std::sqrt(s2);
+                if (red == aux) red = 0.5 * p->jacobi_tol;
+            }
+            std::swap(hs.T, hs.Tn);
+            counter++;
+            if (p->max_sweeps > 0 && counter > p->max_sweeps) {
+                set_error("Jacobi loop exceeded max_sweeps=%lld", (long long)p->max_sweeps);
+                return SHG_ENOCONV;
+            }
+        }
+        info.sweeps = counter;
+        info.residual = red;
+        info.converged_in_one = counter == 1;
+        return SHG_OK;
+    }
+    DirectTables& dt = *static_cast<DirectTables*>(hs.solver);
+    // the reference's first check: the change of one sweep from the current T
+    double r0sq = 0;
+    int rc = dt.fp64 ? direct_step<double>(h, c, hs, dt, &r0sq, nullptr)
+                     : direct_step<float>(h, c, hs, dt, &r0sq, nullptr);
+    if (rc) return rc;
+    if (std::sqrt(r0sq) < p->jacobi_tol) {
+        ProfScope ps(h, "heat_jacobi_sweep");
+        SHG_HIP(launch_heat_sweep(h->stream, c, hs.T, hs.Tn, hs.Q, hs.part, false));
+        std::swap(hs.T, hs.Tn);
+        info.sweeps = 1;
+        info.residual = std::sqrt(r0sq);
+        info.converged_in_one = true;
+        return SHG_OK;
+    }
+    const double tol = p->heat_tol_K > 0 ? p->heat_tol_K : 1e-8;
+    double mc = INFINITY;
+    int it = 0;
+    for (; it < 40; it++) {
+        rc = dt.fp64 ? direct_step<double>(h, c, hs, dt, nullptr, &mc) : direct_step<float>(h, c, hs, dt, nullptr, &mc);
+        if (rc) return rc;
+        if (!std::isfinite(mc)) {
+            set_error("heat: non-finite correction");
+            return SHG_ENAN;
+        }
+        if (mc < tol) break;
+    }
+    if (mc >= tol) {
+        set_error("heat: defect correction did not reach %g K (last update %g K)", tol, mc);
+        return SHG_ENOCONV;
+    }
+    // report the reference's measure (one-sweep change) at the solution
+    double rs = 0;
+    rc = dt.fp64 ? direct_step<double>(h, c, hs, dt, &rs, nullptr) : direct_step<float>(h, c, hs, dt, &rs, nullptr);
+    if (rc) return rc;
+    info.sweeps = 0;
+    info.residual = std::sqrt(rs);
+    info.converged_in_one = false;
+    info.refinements = it + 1;
+    return SHG_OK;
+}
+
+int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc) {
+    const long n3 = (long)hs.nx * hs.ny * hs.nzh;
+    {
+        ProfScope ps(h, "heat_dk");
+        SHG_HIP(launch_heat_dk(h->stream, hs.T, hs.dk, hs.T32, n3, p->crystal, p->lambda_p_um, p->Lambda_um,
+                               p->therm_exp_a, p->therm_exp_b));
+        SHG_HIP(launch_heat_dkbase(h->stream, hs.dk, hs.dkbase, hs.nx * hs.ny, hs.nzh, p->nz));
+    }
+    // temperature at the grid centre of heat slice 0 (src/headers/Efields.h:88,124)
+    const size_t ic = (size_t)(hs.ny / 2) * hs.nx + hs.nx / 2;
+    return read_scalar(h, hs.T + ic, Tc);
+}
+
+int heat_finish(shg_handle* h, const shg_params* p, HeatState& hs, shg_result* r) {
+    const long n3 = (long)hs.nx * hs.ny * hs.nzh;
+    SHG_HIP(launch_max(h->stream, hs.T, n3, hs.part, std::min(hs.nblocks, 1024)));
+    SHG_HIP(launch_reduce(h->stream, hs.part, std::min(hs.nblocks, 1024), true, hs.red + 2));
+    double mx = 0;
+    if (int rc = read_scalar(h, hs.red + 2, &mx)) return rc;
+    r->T_max_C = mx;
+    r->peak_dT_K = mx - p->T_peltier_C;
+    if (r->temperature_out)
+        SHG_HIP(hipMemcpy(r->temperature_out, hs.T, sizeof(double) * n3, hipMemcpyDeviceToHost));
+    return SHG_OK;
+}
+
+}  // namespace shg

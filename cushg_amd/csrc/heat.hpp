@@ -21,6 +21,8 @@ struct HeatState {
     double* dkbase = nullptr;  // running-integral base per heat slice
     double* pol = nullptr;     // poling phases [nz] or null
     double* red = nullptr;     // reduction slots
+    double* part = nullptr;    // per-block partials
+    int nblocks = 0;
     void* solver = nullptr;    // direct-solver tables (opaque)
     bool converged_in_one = false;
 };
@@ -29,6 +31,7 @@ struct HeatSolveInfo {
     long long sweeps = 0;
     double residual = 0;
     bool converged_in_one = false;
+    int refinements = 0;
 };
 
 size_t heat_bytes(const shg_params* p);
@@ -37,5 +40,6 @@ int heat_init(shg_handle* h, const shg_params* p, HeatState& hs);
 int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo& info);
 int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc);
 int heat_finish(shg_handle* h, const shg_params* p, HeatState& hs, shg_result* r);
+void heat_release(HeatState& hs);
 
 }  // namespace shg

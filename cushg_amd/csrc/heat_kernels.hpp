@@ -1,0 +1,49 @@
+// heat_kernels.hpp -- coefficient structs and launchers of kernels_heat.hip.
+#pragma once
+
+#include "common.hpp"
+
+namespace shg {
+
+// Constants of the open-top Jacobi sweep (src/headers/Tfield.h:99-106)
+struct HeatCoef {
+    int nx, ny, nz;
+    double kth, Tinf;
+    double idx2, idy2, idz2, L2;
+    double Bix, Biy, Biz;
+    double ax, ay, az, bx, by, bz;
+    double av, bv, cv, V, ah, bh, ch, H, ad, bd, cd, D;
+    double AV, BV, CV, BiD, GV;
+};
+
+// 1D factors of the tensor-product finite-volume form
+//   A = Kx (x) Wy (x) Wz + Wx (x) Ky (x) Wz + Wx (x) Wy (x) Kz
+// (kx/ky/kz: diagonals of K; wx/wy/wz: control-volume widths; lx/ly/lz:
+// eigenvalues of W^-1/2 K W^-1/2).  y runs over the unknowns y = 1..ny-1.
+struct HeatAxes {
+    const double *kx, *ky, *kz;
+    const double *wx, *wy, *wz;
+    const double *lx, *ly, *lz;
+};
+
+HeatCoef heat_coef(const shg_params* p, int nzh);
+int heat_grid_blocks(const HeatCoef& c);
+
+hipError_t launch_heat_sweep(hipStream_t st, const HeatCoef& c, const double* Ti, double* Tf, const float* Q,
+                             double* part_sq, bool want_norm);
+template <typename R>
+hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* T, const float* Q,
+                             R* u, double* part_sq);
+template <typename R>
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nx, int nyu, int nz);
+template <typename R>
+hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
+                            double* part_max);
+hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out);
+hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel);
+hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float* T32, long n, int crystal, double lp,
+                          double Lambda, double ea, double eb);
+hipError_t launch_heat_dkbase(hipStream_t st, const double* dk, double* base, int n2, int nzh, int nz);
+hipError_t launch_max(hipStream_t st, const double* a, long n, double* part, int nblocks);
+
+}  // namespace shg

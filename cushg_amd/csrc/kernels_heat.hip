@@ -1,0 +1,298 @@
+// kernels_heat.hip -- THERMAL kernels: the reference's Jacobi sweep (as a
+// sweep and as an FP64 residual), the pointwise stages of the
+// fast-diagonalisation solve, dk(T) and the running dk integral base, and
+// the heat source clear.
+#include "heat_kernels.hpp"
+
+namespace shg {
+
+// ------------------------------------------------------------------
+// One application of the Jacobi map of src/headers/Tfield.h:93-201 at
+// node (x, y, z), y >= 1 (the y = 0 plane is the Dirichlet oven).
+// ------------------------------------------------------------------
+__device__ __forceinline__ double jacobi_node(const HeatCoef& c, const double* __restrict__ Ti,
+                                              const float* __restrict__ Q, int x, int y, int z) {
+    const int NX = c.nx, NY = c.ny, NZ = c.nz;
+    const size_t sx = 1, sy = (size_t)NX, sz = (size_t)NX * NY;
+    const size_t i = (size_t)z * sz + (size_t)y * sy + x;
+    const bool xi = x > 0 && x < NX - 1, yi = y < NY - 1, zi = z > 0 && z < NZ - 1;
+    const bool xe = !xi, ze = !zi;
+    const size_t ixn = x == 0 ? i + sx : i - sx;  // inward x neighbour
+    const size_t izn = z == 0 ? i + sz : i - sz;  // inward z neighbour
+    const double Tinf = c.Tinf;
+    if (xi && yi && zi)  // inner points, :113-115
+        return c.L2 * ((Ti[i + sx] + Ti[i - sx]) * c.idx2 + (Ti[i + sy] + Ti[i - sy]) * c.idy2 +
+                       (Ti[i + sz] + Ti[i - sz]) * c.idz2 + (double)Q[i] / c.kth);
+    if (xe && yi && zi)  // left/right faces, :119-125
+        return (Ti[ixn] + c.ax * (Ti[i + sy] + Ti[i - sy]) + c.bx * (Ti[i + sz] + Ti[i - sz]) + c.Bix * Tinf) /
+               (1 + 2 * c.ax + 2 * c.bx + c.Bix);
+    if (xi && !yi && zi)  // top face, :127-129
+        return (Ti[i - sy] + c.ay * (Ti[i + sx] + Ti[i - sx]) + c.by * (Ti[i + sz] + Ti[i - sz]) + c.Biy * Tinf) /
+               (1 + 2 * c.ay + 2 * c.by + c.Biy);
+    if (xi && yi && ze)  // back/front faces, :131-137
+        return (Ti[izn] + c.az * (Ti[i + sx] + Ti[i - sx]) + c.bz * (Ti[i + sy] + Ti[i - sy]) + c.Biz * Tinf) /
+               (1 + 2 * c.az + 2 * c.bz + c.Biz);
+    if (xe && yi && ze)  // vertical sides, :143-157
+        return (c.av * Ti[ixn] + c.bv * Ti[izn] + c.cv * (Ti[i + sy] + Ti[i - sy]) + 0.5 * Tinf * (c.Bix + c.Biz)) * c.V;
+    if (xi && !yi && ze)  // horizontal sides, :161-167
+        return (c.ah * Ti[i - sy] + c.bh * Ti[izn] + c.ch * (Ti[i - sx] + Ti[i + sx]) + 0.5 * Tinf * (c.Biy + c.Biz)) *
+               c.H;
+    if (xe && !yi && zi)  // depth sides, :171-177
+        return (c.ad * Ti[i - sy] + c.bd * Ti[ixn] + c.cd * (Ti[i + sz] + Ti[i - sz]) + 0.5 * Tinf * (c.Bix + c.Biy)) *
+               c.D;
+    // vertices, :181-196
+    return (c.AV * Ti[ixn] + c.BV * Ti[i - sy] + c.CV * Ti[izn] + Tinf * c.BiD) * c.GV;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int n = blockDim.x * blockDim.y;
+    red[t] = v;
+    __syncthreads();
+    for (int s = n / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ double block_max(double v, double* red) {
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int n = blockDim.x * blockDim.y;
+    red[t] = v;
+    __syncthreads();
+    for (int s = n / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = fmax(red[t], red[t + s]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Jacobi sweep Tf = F(Ti) (Tfield::upDate); with `want_norm`, per-block
+// partial sums of (Tf - Ti)^2 (Tfield::checkConvergence, :366-376).
+// block (64, 4), grid (nx/64, ny/4, nz)
+__global__ __launch_bounds__(256) void k_heat_sweep(HeatCoef c, const double* __restrict__ Ti,
+                                                    double* __restrict__ Tf, const float* __restrict__ Q,
+                                                    double* __restrict__ part_sq, int want_norm) {
+    __shared__ double red[256];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    double d2 = 0.0;
+    if (x < c.nx && y < c.ny) {
+        const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
+        if (y == 0) {
+            Tf[i] = Ti[i];
+        } else {
+            const double v = jacobi_node(c, Ti, Q, x, y, z);
+            d2 = (v - Ti[i]) * (v - Ti[i]);
+            Tf[i] = v;
+        }
+    }
+    if (want_norm) {
+        const double s = block_sum(d2, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            part_sq[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// FP64 residual of the steady state: delta = F(T) - T (the reference's
+// one-sweep change) and the finite-volume residual r = A_ii * delta, written
+// scaled by W^-1/2 into the unknown layout u[z][y-1][x] that the
+// fast-diagonalisation transforms consume.  Partial sums of delta^2.
+template <typename R>
+__global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, const double* __restrict__ T,
+                                                    const float* __restrict__ Q, R* __restrict__ u,
+                                                    double* __restrict__ part_sq) {
+    __shared__ double red[256];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    double d2 = 0.0;
+    if (x < c.nx && y >= 1 && y < c.ny) {
+        const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
+        const double d = jacobi_node(c, T, Q, x, y, z) - T[i];
+        d2 = d * d;
+        const int j = y - 1;
+        const double wx = a.wx[x], wy = a.wy[j], wz = a.wz[z];
+        const double aii = a.kx[x] * wy * wz + wx * a.ky[j] * wz + wx * wy * a.kz[z];
+        u[((size_t)z * (c.ny - 1) + j) * c.nx + x] = (R)(aii * d * rsqrt(wx * wy * wz));
+    }
+    const double s = block_sum(d2, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        part_sq[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// spectral division by lambda_x + lambda_y + lambda_z
+template <typename R>
+__global__ void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx, const double* __restrict__ ly,
+                            const double* __restrict__ lz, int nx, int nyu, int nz) {
+    const long n = (long)nx * nyu * nz;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
+        const int x = (int)(i % nx);
+        const long r = i / nx;
+        const int j = (int)(r % nyu);
+        const int z = (int)(r / nyu);
+        v[i] = (R)((double)v[i] / (lx[x] + ly[j] + lz[z]));
+    }
+}
+
+// T += W^-1/2 u (the correction), partial max |correction|
+template <typename R>
+__global__ __launch_bounds__(256) void k_fd_update(HeatCoef c, HeatAxes a, const R* __restrict__ u,
+                                                   double* __restrict__ T, double* __restrict__ part_max) {
+    __shared__ double red[256];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    double m = 0.0;
+    if (x < c.nx && y >= 1 && y < c.ny) {
+        const int j = y - 1;
+        const double corr = (double)u[((size_t)z * (c.ny - 1) + j) * c.nx + x] * rsqrt(a.wx[x] * a.wy[j] * a.wz[z]);
+        T[((size_t)z * c.ny + y) * c.nx + x] += corr;
+        m = fabs(corr);
+    }
+    const double s = block_max(m, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        part_max[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part, int n, int op,
+                                                double* __restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc = op ? fmax(acc, part[i]) : acc + part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = op ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                                   : red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// T = T_C everywhere, y = 0 plane = T_peltier (Tfield::setTemperature /
+// setBottomOvens, src/headers/Tfield.h:28-61); Q = 0 (setInitialQ)
+__global__ void k_heat_init(double* __restrict__ T, float* __restrict__ Q, int nx, int ny, int nz, double Tc,
+                            double Tpel) {
+    const long n = (long)nx * ny * nz;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
+        const int y = (int)((i / nx) % ny);
+        T[i] = y == 0 ? Tpel : Tc;
+        Q[i] = 0.0f;
+    }
+}
+
+// dk(T) per voxel in FP64 (PhaseMatching::setDKFromTemperature,
+// src/headers/PhaseMatching.h:47-65) and the FP32 copy of T for the optics
+__global__ void k_heat_dk(const double* __restrict__ T, double* __restrict__ dk, float* __restrict__ T32, long n,
+                          int crystal, double lp, double Lambda, double ea, double eb) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
+        const double t = T[i];
+        dk[i] = delta_k(crystal, lp, Lambda, ea, eb, t);
+        T32[i] = (float)t;
+    }
+}
+
+// Base of the running dk integral per heat slice:  the optical slices
+// s in [s0(h), s0(h) + cnt(h)) see  DKint_s = sum_{j<=s} dk[h(j)]
+// = dkbase[h] + s dk[h]  (src/headers/PhaseMatching.h:27-44 summed in O(1)).
+__global__ void k_heat_dkbase(const double* __restrict__ dk, double* __restrict__ base, int n2, int nzh, int nz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    double cum = 0.0;
+    for (int h = 0; h < nzh; h++) {
+        const long long s0 = ((long long)h * nz + nzh - 1) / nzh;
+        const long long s1 = ((long long)(h + 1) * nz + nzh - 1) / nzh;
+        const double d = dk[(size_t)h * n2 + i];
+        base[(size_t)h * n2 + i] = cum - (double)(s0 - 1) * d;
+        cum += (double)(s1 - s0) * d;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_max(const double* __restrict__ a, long n, double* __restrict__ part) {
+    __shared__ double red[256];
+    double m = -1e300;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) m = fmax(m, a[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------
+static dim3 heat_grid(const HeatCoef& c) { return dim3((c.nx + 63) / 64, (c.ny + 3) / 4, c.nz); }
+int heat_grid_blocks(const HeatCoef& c) {
+    const dim3 g = heat_grid(c);
+    return (int)(g.x * g.y * g.z);
+}
+
+hipError_t launch_heat_sweep(hipStream_t st, const HeatCoef& c, const double* Ti, double* Tf, const float* Q,
+                             double* part_sq, bool want_norm) {
+    hipLaunchKernelGGL(k_heat_sweep, heat_grid(c), dim3(64, 4), 0, st, c, Ti, Tf, Q, part_sq, want_norm ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* T, const float* Q,
+                             R* u, double* part_sq) {
+    hipLaunchKernelGGL((k_heat_resid<R>), heat_grid(c), dim3(64, 4), 0, st, c, a, T, Q, u, part_sq);
+    return hipGetLastError();
+}
+template hipError_t launch_heat_resid<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*,
+                                             const float*, float*, double*);
+template hipError_t launch_heat_resid<double>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*,
+                                              const float*, double*, double*);
+
+template <typename R>
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nx, int nyu, int nz) {
+    const long n = (long)nx * nyu * nz;
+    long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL((k_fd_divide<R>), dim3((unsigned)nb), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nx, nyu, nz);
+    return hipGetLastError();
+}
+template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int);
+template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxes&, int, int, int);
+
+template <typename R>
+hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
+                            double* part_max) {
+    hipLaunchKernelGGL((k_fd_update<R>), heat_grid(c), dim3(64, 4), 0, st, c, a, u, T, part_max);
+    return hipGetLastError();
+}
+template hipError_t launch_fd_update<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const float*, double*,
+                                            double*);
+template hipError_t launch_fd_update<double>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*, double*,
+                                             double*);
+
+hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out) {
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, part, n, is_max ? 1 : 0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel) {
+    hipLaunchKernelGGL(k_heat_init, dim3(2048), dim3(256), 0, st, T, Q, nx, ny, nz, Tc, Tpel);
+    return hipGetLastError();
+}
+
+hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float* T32, long n, int crystal, double lp,
+                          double Lambda, double ea, double eb) {
+    hipLaunchKernelGGL(k_heat_dk, dim3(2048), dim3(256), 0, st, T, dk, T32, n, crystal, lp, Lambda, ea, eb);
+    return hipGetLastError();
+}
+
+hipError_t launch_heat_dkbase(hipStream_t st, const double* dk, double* base, int n2, int nzh, int nz) {
+    hipLaunchKernelGGL(k_heat_dkbase, dim3((n2 + 255) / 256), dim3(256), 0, st, dk, base, n2, nzh, nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_max(hipStream_t st, const double* a, long n, double* part, int nblocks) {
+    hipLaunchKernelGGL(k_max, dim3(nblocks), dim3(256), 0, st, a, n, part);
+    return hipGetLastError();
+}
+
+}  // namespace shg

@@ -277,10 +277,20 @@ static size_t mixed_bytes(const MixedHost& m) {
     return b;
 }
 
+static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& ps,
+                          const std::vector<shg_result*>& rs, std::vector<MixedHost>& ms);
+
 int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs) {
+    std::vector<MixedHost> ms(ps.size());
+    const int rc = run_mixed_impl(h, ps, rs, ms);
+    for (auto& m : ms) heat_release(m.heat);
+    return rc;
+}
+
+static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& ps,
+                          const std::vector<shg_result*>& rs, std::vector<MixedHost>& ms) {
     const auto t0 = std::chrono::steady_clock::now();
     const int n = (int)ps.size();
-    std::vector<MixedHost> ms(n);
     size_t total = 0;
     for (int i = 0; i < n; i++) {
         MixedHost& m = ms[i];

@@ -55,6 +55,7 @@ struct shg_handle {
     shg::Arena arena;
     std::map<int, float4*> tw32;
     std::map<int, double2*> tw64;
+    void* blas = nullptr;         // rocBLAS handle (heat solver), created lazily
     double* d_scalars = nullptr;  // reduction slots
     double* d_partials = nullptr;
     int n_partials = 0;

@@ -2,12 +2,6 @@
 #include "heat.hpp"
 #include "runtime.hpp"
 namespace shg {
-size_t heat_bytes(const shg_params*) { return 0; }
-int heat_alloc(shg_handle*, const shg_params*, HeatState&) { set_error("THERMAL not built"); return SHG_EUNSUPPORTED; }
-int heat_init(shg_handle*, const shg_params*, HeatState&) { return SHG_EUNSUPPORTED; }
-int heat_solve(shg_handle*, const shg_params*, HeatState&, HeatSolveInfo&) { return SHG_EUNSUPPORTED; }
-int heat_prepare_optics(shg_handle*, const shg_params*, HeatState&, double*) { return SHG_EUNSUPPORTED; }
-int heat_finish(shg_handle*, const shg_params*, HeatState&, shg_result*) { return SHG_EUNSUPPORTED; }
 int run_plain(shg_handle*, const shg_params*, shg_result*) { set_error("FP64 plain not built"); return SHG_EUNSUPPORTED; }
 int run_adjoint(shg_handle*, const shg_params*, int, double*, shg_result*) { set_error("adjoint not built"); return SHG_EUNSUPPORTED; }
 }
