@@ -1,20 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the SSFM propagation path (one MI355X).
+"""Benchmark of the SSFM propagation path on one MI355X.
 
-Metric: grid-point * z-steps per second (sum over instances of nx*ny*steps
-divided by wall time), FP32-mixed form, inputs resident on the device (the
-pump is initialised on the GPU; nothing crosses PCIe in the timed region
-except the scalar results).  A "step" of this benchmark is one complete
-run of the workload (all thermo-optic passes and heat solves included).
+Metric: grid-point * z-steps per second = sum over instances of
+nx*ny*(z-steps executed) / wall time, FP32-mixed batched form, everything
+resident on the device (the pump is initialised on the GPU; only scalar
+results cross PCIe in the timed region).  One benchmark "step" is one
+complete run of the workload: every thermo-optic pass, heat solve and
+z-march of every instance.
 
-Default workload (THERMAL, 256x256, 1000 z-steps, 50 mm, 10 W, 40 um,
-heat grid 256x256x100, oven 40 C): see cushg_amd.configs.config2.
+Default workload (the configuration the metric's target is quoted on):
+config 5's batch -- 16 THERMAL instances, 512x512 grid, 4000 z-steps,
+50 mm MgO:PPLN, 30 W, heat grid 512x512x400, over seeded (waist, focus,
+oven temperature) triples (cushg_amd.configs.config5_batch).
 
-Prints one JSON line.  Extra keys: "roofline" of the dominant kernel
-(algorithmic bytes per launch / HIP-event time of the launch on the
-library's stream), "cpu_baseline" (the FP64 oracle on a bounded sample of
-the same workload, host cores), "fp64_plain" (the same workload in the
-plain FP64 GPU form) and the accuracy of the run against the oracle.
+The JSON line also carries:
+  roofline      dominant kernel: algorithmic bytes per launch / average
+                launch time from HIP events on the library's stream;
+                `traffic` = HBM bytes per launch from the committed
+                rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/).
+  fp64_plain    the same instances in the plain FP64 GPU form (which runs
+                one instance at a time), timed on the first instance.
+  cpu_baseline  the FP64 oracle (OpenMP, all host threads) on a bounded
+                sample of the first instance.
 """
 from __future__ import annotations
 
@@ -32,27 +39,26 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per grid point of one launch
 BYTES_PER_PT = {
     "rowpass": 32.0,          # read P,S + write P,S (FP32 complex)
-    "rowpass_thermal": 60.0,  # + T (4) + dk, dk-base (16) + Q read/write (8)
+    "rowpass_thermal": 60.0,  # + (n_p, n_s) 8 + dk, dk-base 16 + Q read/write 8
     "colpass": 32.0,          # both fields: read + write
 }
 
 
 def workload(name: str):
-    from cushg_amd import abi, configs
+    from cushg_amd import configs
+    if name == "cfg5b16":
+        return configs.config5_batch(16), ("16 x THERMAL 512x512, 4000 z-steps, 50 mm MgO:PPLN, 30 W, "
+                                           "heat 512x512x400, seeded waist/focus/oven")
+    if name == "cfg5":
+        return [configs.config5()], "THERMAL 512x512, 4000 z-steps, 50 mm MgO:PPLN, 30 W, 50 um, heat 512x512x400"
     if name == "cfg2":
         return [configs.config2()], "THERMAL 256x256, 1000 z-steps, 50 mm sPPLT, 10 W, 40 um, heat 256x256x100, oven 40 C"
     if name == "cfg2-optics":
         p = configs.config2()
         p.thermal = 0
         return [p], "256x256, 1000 z-steps, 50 mm sPPLT, 10 W, 40 um, THERMAL off"
-    if name == "cfg5":
-        return [configs.config5()], "THERMAL 512x512, 4000 z-steps, 50 mm PPLN, 30 W, 50 um, heat 512x512x400"
-    if name == "cfg5-optics":
-        p = configs.config5()
-        p.thermal = 0
-        return [p], "512x512, 4000 z-steps, 50 mm PPLN, 30 W, 50 um, THERMAL off"
     if name == "cfg3":
-        return configs.config3(seed=0, n=64), "64-instance sweep, grids 128..512, 500-2000 z-steps"
+        return configs.config3(seed=0, n=64), "64-instance sweep, grids 128..512, 500-2000 z-steps, THERMAL off"
     if name == "cfg1":
         return [configs.config1()], "64x64, 100 z-steps, 30 mm sPPLT, 1 W, 30 um"
     raise SystemExit(f"unknown workload {name}")
@@ -63,15 +69,29 @@ def pts_steps(ps, results):
 
 
 def run_once(h, ps):
+    from cushg_amd import shg
     rc, res, _ = h.run_batch(ps)
     if rc != 0:
-        from cushg_amd import shg
         raise RuntimeError(f"run failed: {rc} {shg.last_error()}")
     return res
 
 
+def committed_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("kernels", {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 def kernel_roofline(h, ps):
-    """Dominant kernel's achieved algorithmic bandwidth from HIP events."""
+    """Dominant kernel's achieved algorithmic bandwidth: its algorithmic
+    bytes summed over every launch of a profiled run (each launch covers the
+    instances still marching in it), divided by the kernel's total time from
+    HIP events recorded on the library's stream."""
     h.set_profiling(True)
     res = run_once(h, ps)
     kt = h.kernel_times()
@@ -79,68 +99,96 @@ def kernel_roofline(h, ps):
     if not kt:
         return None, kt
     name, (ms, n) = max(kt.items(), key=lambda kv: kv[1][0])
-    base = "rowpass_thermal" if name.endswith("_thermal") else name.split("_")[0]
+    parts = name.split("_")
+    base = "rowpass_thermal" if name.endswith("_thermal") else parts[0]
     bpp = BYTES_PER_PT.get(base)
     if bpp is None:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}, kt
-    npts = sum(p.nx * p.ny for p in ps)
-    bytes_per_launch = bpp * npts
-    avg_ms = ms / n
-    ach = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    size = int(parts[1])
+    total = 0.0
+    for p, r in zip(ps, res):
+        if base.startswith("row") and p.nx == size:
+            total += (r.steps_done + r.outer_iters) * p.nx * p.ny * bpp  # nz + 1 row passes per march
+        elif base == "colpass" and p.ny == size:
+            total += r.steps_done * p.nx * p.ny * bpp
+    ach = total / (ms * 1e-3) / 1e9
+    tr = committed_traffic(name)
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": round(avg_ms * 1e3, 3), "launches": n}, kt
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr["bytes_per_launch"] if tr else None,
+            "algorithmic_bytes_per_launch": round(total / n, 1), "avg_launch_us": round(ms / n * 1e3, 3),
+            "launches": n, "timer": "hipEvent on the library stream"}, kt
 
 
 def cpu_baseline(ps, budget_s: float):
     """FP64 oracle (OpenMP, all host threads) on a bounded sample of the
-    workload: the first instance with its z-steps cut to fit ~budget_s."""
+    workload: the first instance at its full transverse grid, one
+    thermo-optic pass (heat solve + z-march), the heat grid cut to at most
+    16 slices and the z-steps cut so that the run takes about budget_s."""
     from oracle import oracle
     p = ps[0].copy()
+    p.max_outer = 1
+    if p.thermal:
+        p.nz_heat = min(p.nz_heat if p.nz_heat > 0 else p.nz, 16)
     threads = oracle.get_threads()
-    # calibrate on a short march
     q = p.copy()
-    q.nz = 8
-    if q.thermal:
-        q.nz_heat = min(q.nz_heat if q.nz_heat > 0 else q.nz, 8)
+    q.nz = 16
     t = time.perf_counter()
     r, _ = oracle.run(q, fields=False)
     dt = max(time.perf_counter() - t, 1e-3)
-    per = dt / max(r.steps_done, 1)
-    nz = int(max(8, min(p.nz, budget_s / per / max(r.outer_iters, 1))))
+    nz = int(max(16, min(p.nz, q.nz * budget_s / dt)))
     q = p.copy()
     q.nz = nz
-    if q.thermal:
-        q.nz_heat = max(2, min(q.nz_heat if q.nz_heat > 0 else q.nz, nz))
     t = time.perf_counter()
     r, _ = oracle.run(q, fields=False)
     dt = time.perf_counter() - t
     return {"value": round(q.nx * q.ny * r.steps_done / dt, 1), "unit": "grid-pt*z-steps/s",
             "cores": threads, "kind": "port",
-            "sample": f"{q.nx}x{q.ny}, {r.steps_done} z-steps ({r.outer_iters} thermo-optic passes), "
-                      f"nz_heat={q.nz_heat if q.thermal else 0}, {dt:.1f} s"}
+            "sample": f"first instance, {q.nx}x{q.ny}, one thermo-optic pass of {r.steps_done} z-steps"
+                      + (f" with a {q.nx}x{q.ny}x{q.nz_heat} heat solve" if q.thermal else "")
+                      + f", {dt:.1f} s"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--workload", default="cfg5b16")
+    ap.add_argument("--nz", type=int, default=0, help="override z-steps (profiling runs)")
+    ap.add_argument("--max-outer", type=int, default=0, help="override thermo-optic passes (profiling runs)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "launches": 3}))
+        return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the batch shards by "
-              f"instance across GPUs by design, but no multi-GPU figure is produced.", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU (asked for {args.gpus}); a batch shards by instance "
+              f"across GPUs by design, but no multi-GPU figure is measured here.", file=sys.stderr)
         sys.exit(2)
 
     from cushg_amd import abi, shg
 
     ps, desc = workload(args.workload)
+    for p in ps:
+        if args.nz:
+            p.nz = args.nz
+        if args.max_outer:
+            p.max_outer = args.max_outer
+    print(f"bench: {args.workload} ({len(ps)} instances) warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
     h = shg.Handle(0)
     for _ in range(args.warmup):
         res = run_once(h, ps)
@@ -150,27 +198,28 @@ def main():
     dt = time.perf_counter() - t0
     work = pts_steps(ps, res)
     value = work * args.steps / dt
+    print(f"bench: timed {dt:.2f} s", file=sys.stderr, flush=True)
 
-    roof, kt = kernel_roofline(h, ps)
+    roof, kt = (None, {}) if args.no_roofline else kernel_roofline(h, ps)
 
     extra = {}
-    if not args.no_fp64 and len(ps) == 1:
+    if not args.no_fp64:
         q = ps[0].copy()
         q.precision = abi.PREC_FP64_PLAIN
-        try:
-            r64 = run_once(h, [q])
-            t = time.perf_counter()
-            r64 = run_once(h, [q])
-            d64 = time.perf_counter() - t
-            v64 = pts_steps([q], r64) / d64
-            extra["fp64_plain"] = {"value": round(v64, 1), "ms_per_step": round(d64 * 1e3, 3),
-                                   "eta": r64[0].eta}
-            extra["speedup_vs_fp64_plain"] = round(value / v64, 2)
-        except Exception as e:  # plain form unavailable: say so
-            extra["fp64_plain"] = {"error": str(e)}
+        run_once(h, [q])
+        t = time.perf_counter()
+        r64 = run_once(h, [q])
+        d64 = time.perf_counter() - t
+        v64 = pts_steps([q], r64) / d64
+        extra["fp64_plain"] = {"value": round(v64, 1), "unit": "grid-pt*z-steps/s",
+                               "sample": "first instance (the plain form runs instances one at a time)",
+                               "ms_per_instance": round(d64 * 1e3, 1), "eta": r64[0].eta,
+                               "eta_rel_err_fp32_vs_fp64": abs(res[0].eta - r64[0].eta) / abs(r64[0].eta)}
+        extra["speedup_vs_fp64_plain"] = round(value / v64, 2)
+        print(f"bench: fp64 plain {d64:.2f} s", file=sys.stderr, flush=True)
 
     line = {
-        "metric": "grid-pt*z-steps/s (FP32-mixed form)",
+        "metric": "grid-pt*z-steps/s (FP32-mixed batched form)",
         "value": round(value, 1),
         "unit": "grid-pt*z-steps/s",
         "n_gpus": 1,
@@ -180,11 +229,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (fields/FFT/RK4) + fp64 (phase, reductions, heat)",
+        "dtype": "fp32",
+        "dtype_detail": "fields, FFTs, RK4 in fp32; dk*z phase, reductions and heat residual in fp64",
         "data": "synthetic (Gaussian pump from parameters)",
         "config": {"workload": f"{args.workload}: {desc}", "instances": len(ps),
                    "z_steps_per_run": int(sum(r.steps_done for r in res)),
-                   "eta": res[0].eta, "peak_dT_K": res[0].peak_dT_K, "outer_iters": res[0].outer_iters},
+                   "eta": [round(r.eta, 9) for r in res[:4]],
+                   "peak_dT_K": [round(r.peak_dT_K, 6) for r in res[:4]],
+                   "outer_iters": [r.outer_iters for r in res]},
         "roofline": roof,
     }
     line.update(extra)

@@ -121,6 +121,23 @@ def config5() -> ShgParams:
     return p
 
 
+def config5_batch(n: int = 16, seed: int = 5):
+    """Config 5's batch: n THERMAL 512x512x4000 instances (heat 512x512x400)
+    over seeded (waist, focus, oven temperature) triples around the nominal
+    point: waist U(40, 60) um, focus U(0.4 L, 0.6 L), oven U(38, 42) C, the
+    grating fixed at the 40 C first-order period."""
+    base = config5()
+    r = splitmix64(seed)
+    out = []
+    for _ in range(n):
+        p = base.copy()
+        p.waist_um = 40.0 + 20.0 * next(r)
+        p.focal_um = p.lz_um * (0.4 + 0.2 * next(r))
+        p.T_C = p.T_peltier_C = 38.0 + 4.0 * next(r)
+        out.append(p)
+    return out
+
+
 def scaled(p: ShgParams, n: int | None = None, nz: int | None = None, nz_heat: int | None = None) -> ShgParams:
     """Same physics on a smaller grid (for parity tests the oracle finishes fast)."""
     q = p.copy()

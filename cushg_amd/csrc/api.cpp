@@ -15,6 +15,7 @@ const char* last_error();
 int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs);
 int run_plain(shg_handle* h, const shg_params* p, shg_result* r);
 int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r);
+void heat_cache_free(shg_handle* h);
 }  // namespace shg
 
 using namespace shg;
@@ -194,13 +195,13 @@ int shg_handle_destroy(shg_handle* h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     prof_collect(h);
-    for (auto& kv : h->tw32) (void)hipFree(kv.second);
     for (auto& kv : h->tw64) (void)hipFree(kv.second);
     for (auto e : h->event_pool) (void)hipEventDestroy(e);
     if (h->arena.base) (void)hipFree(h->arena.base);
     if (h->d_scalars) (void)hipFree(h->d_scalars);
     if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
+    heat_cache_free(h);
     (void)hipStreamDestroy(h->stream);
     delete h;
     return SHG_OK;

@@ -18,10 +18,12 @@
 // FP32 accuracy: a constant rounded to FP32 (sqrt(1/2), sqrt(3)/2, a
 // twiddle) carries a fixed relative error of up to 3e-8 that every transform
 // applies again to the same modes, so over thousands of SSFM slices it
-// accumulates coherently (a systematic power loss of ~3e-8 per FFT round
-// trip with plain FP32 constants).  Here every constant is split into
-// hi + lo FP32 parts and applied with FMAs (t*c = fma(t, hi, t*lo)), which
-// leaves only data-dependent rounding that does not accumulate coherently.
+// accumulates coherently (a systematic power drift of ~3e-8 per FFT round
+// trip with plain FP32 constants).  Here every product with a constant is
+// correctly rounded: the radix constants are split into hi + lo FP32 parts
+// applied by one FMA (t*c = fma(t, hi, t*lo)), and twiddles are applied in
+// FP64; what remains is data-dependent rounding that does not accumulate
+// coherently.
 #pragma once
 
 #include "common.hpp"
@@ -37,21 +39,22 @@ template <int N> struct FftShape {
 
 __device__ __forceinline__ constexpr int fpad(int i) { return i + (i >> 3); }
 
-// FP32 twiddles are float4 {hi.x, hi.y, lo.x, lo.y}; FP64 twiddles double2.
+// Twiddle tables are FP64 for both precisions.  An FP32 point is multiplied
+// by its FP64 twiddle in FP64 and rounded once: the product is correctly
+// rounded, with no fixed per-twiddle error (an FP32 twiddle, even split
+// into hi + lo parts, leaves one: the lo correction is absorbed when the two
+// partial products have similar size, e.g. at 30/60 degree angles).  FP64
+// FMAs issue at the rate of (non-packed) FP32 ones on gfx950.
 template <typename C> struct TwTraits;
-template <> struct TwTraits<float2> { using TW = float4; };
+template <> struct TwTraits<float2> { using TW = double2; };
 template <> struct TwTraits<double2> { using TW = double2; };
 template <typename C> using tw_t = typename TwTraits<C>::TW;
 
-template <bool INV> __device__ __forceinline__ float2 twmul(float2 x, const float4* __restrict__ tw, int m) {
-    const float4 w = tw[m];
-    const float wy = INV ? -w.y : w.y, wly = INV ? -w.w : w.w;
-    const float inr = x.x * w.z - x.y * wly;
-    const float ini = x.x * wly + x.y * w.z;
-    float2 r;
-    r.x = __builtin_fmaf(x.x, w.x, __builtin_fmaf(-x.y, wy, inr));
-    r.y = __builtin_fmaf(x.x, wy, __builtin_fmaf(x.y, w.x, ini));
-    return r;
+template <bool INV> __device__ __forceinline__ float2 twmul(float2 x, const double2* __restrict__ tw, int m) {
+    const double2 w = tw[m];
+    const double wy = INV ? -w.y : w.y;
+    const double xr = x.x, xi = x.y;
+    return make_float2((float)(xr * w.x - xi * wy), (float)(xr * wy + xi * w.x));
 }
 template <bool INV> __device__ __forceinline__ double2 twmul(double2 x, const double2* __restrict__ tw, int m) {
     double2 w = tw[m];
@@ -154,55 +157,72 @@ template <typename C, bool INV, int R> __device__ __forceinline__ void dftR(C* x
 // points to this transform's private LDS buffer (FftShape<N>::LDS entries).
 // Every thread of the workgroup must call it the same number of times
 // (it contains workgroup barriers).
-template <typename C, int N, bool INV>
-__device__ __forceinline__ void fft_dist(C (&v)[8], C* lds, const tw_t<C>* __restrict__ tw, int tp) {
+// NF independent transforms (e.g. the pump and SH rows) sharing the same
+// barriers: v[f] is held by thread tp, buffer f at lds + f*ld.
+template <typename C, int N, bool INV, int NF>
+__device__ __forceinline__ void fft_multi(C (&v)[NF][8], C* lds, int ld, const tw_t<C>* __restrict__ tw, int tp) {
     constexpr int TPR = FftShape<N>::TPR;
     constexpr int M = FftShape<N>::M;
     // pass 1: span 1, radix 8, butterfly t = tp reads x[tp + j*TPR] = v[j]
-    dft8<C, INV>(v);
 #pragma unroll
-    for (int q = 0; q < 8; q++) lds[fpad(8 * tp + q)] = v[q];
+    for (int f = 0; f < NF; f++) {
+        dft8<C, INV>(v[f]);
+#pragma unroll
+        for (int q = 0; q < 8; q++) lds[f * ld + fpad(8 * tp + q)] = v[f][q];
+    }
     __syncthreads();
     if constexpr (M > 1) {
         // pass 2: span 8, radix M, 64 butterflies, twiddle W_{8M}^{jk} = W_N^{8jk}
         constexpr int PER = (64 + TPR - 1) / TPR;
-        C xs[PER][M];
+        C xs[NF][PER][M];
 #pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const int b = tp + i * TPR;
-            if (PER * TPR == 64 || b < 64) {
-                const int k = b & 7;
+        for (int f = 0; f < NF; f++)
 #pragma unroll
-                for (int j = 0; j < M; j++) {
-                    C x = lds[fpad(b + 64 * j)];
-                    if (j > 0) x = twmul<INV>(x, tw, 8 * j * k);
-                    xs[i][j] = x;
+            for (int i = 0; i < PER; i++) {
+                const int b = tp + i * TPR;
+                if (PER * TPR == 64 || b < 64) {
+                    const int k = b & 7;
+#pragma unroll
+                    for (int j = 0; j < M; j++) {
+                        C x = lds[f * ld + fpad(b + 64 * j)];
+                        if (j > 0) x = twmul<INV>(x, tw, 8 * j * k);
+                        xs[f][i][j] = x;
+                    }
                 }
             }
-        }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const int b = tp + i * TPR;
-            if (PER * TPR == 64 || b < 64) {
-                const int k = b & 7;
-                dftR<C, INV, M>(xs[i]);
-                const int base = (b - k) * M + k;
+        for (int f = 0; f < NF; f++)
 #pragma unroll
-                for (int q = 0; q < M; q++) lds[fpad(base + 8 * q)] = xs[i][q];
+            for (int i = 0; i < PER; i++) {
+                const int b = tp + i * TPR;
+                if (PER * TPR == 64 || b < 64) {
+                    const int k = b & 7;
+                    dftR<C, INV, M>(xs[f][i]);
+                    const int base = (b - k) * M + k;
+#pragma unroll
+                    for (int q = 0; q < M; q++) lds[f * ld + fpad(base + 8 * q)] = xs[f][i][q];
+                }
             }
-        }
         __syncthreads();
     }
     // pass 3: span N/8, radix 8, butterfly t = tp, twiddle W_N^{j tp}
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        C x = lds[fpad(tp + j * TPR)];
-        if (j > 0) x = twmul<INV>(x, tw, j * tp);
-        v[j] = x;
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            C x = lds[f * ld + fpad(tp + j * TPR)];
+            if (j > 0) x = twmul<INV>(x, tw, j * tp);
+            v[f][j] = x;
+        }
+        dft8<C, INV>(v[f]);
     }
-    dft8<C, INV>(v);
-    __syncthreads();  // the buffer may be reused by the next call
+    __syncthreads();  // the buffers may be reused by the next call
+}
+
+template <typename C, int N, bool INV>
+__device__ __forceinline__ void fft_dist(C (&v)[8], C* lds, const tw_t<C>* __restrict__ tw, int tp) {
+    fft_multi<C, N, INV, 1>(reinterpret_cast<C(&)[1][8]>(v), lds, 0, tw, tp);
 }
 
 }  // namespace shg

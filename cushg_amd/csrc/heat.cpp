@@ -46,6 +46,10 @@ HeatCoef heat_coef(const shg_params* p, int nzh) {
     c.bx = dx * dx / (2 * dz * dz);
     c.by = dy * dy / (2 * dz * dz);
     c.bz = dz * dz / (2 * dy * dy);
+    c.ikth = 1.0 / kt;
+    c.Fx = 1.0 / (1 + 2 * c.ax + 2 * c.bx + c.Bix);
+    c.Fy = 1.0 / (1 + 2 * c.ay + 2 * c.by + c.Biy);
+    c.Fz = 1.0 / (1 + 2 * c.az + 2 * c.bz + c.Biz);
     c.av = dz / (2 * dx);
     c.bv = dx / (2 * dz);
     c.cv = dx * dz / (4 * dy * dy);
@@ -161,6 +165,88 @@ struct DirectTables {
     bool fp64 = false;
 };
 
+// Per-geometry tables of the direct solver (axis widths, stiffness
+// diagonals, eigenpairs in FP64 and FP32), built once per geometry and kept
+// on the handle: the host eigen-decompositions are O(n^3).
+struct HeatGeom {
+    HeatAxes ax{};
+    void* mem[9] = {};
+    double* q64[3] = {};
+    float* q32[3] = {};
+};
+
+static std::string geom_key(const shg_params* p, const HeatState& hs) {
+    char b[256];
+    snprintf(b, sizeof b, "%d/%d/%d/%.17g/%.17g/%.17g/%.17g", hs.nx, hs.ny, hs.nzh, hs.dx, hs.dy, hs.dz,
+             p->h_conv / p->k_th);
+    return b;
+}
+
+static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& hs) {
+    const std::string key = geom_key(p, hs);
+    auto it = h->heat_cache.find(key);
+    if (it != h->heat_cache.end()) return it->second;
+    // axes: x (Robin both ends), y' = y-1 (Dirichlet below, Robin top), z (Robin both)
+    const double hk = p->h_conv / p->k_th;
+    AxisHost axes[3];
+    const bool ok = axis_build(hs.nx, hs.dx, true, true, false, hk, axes[0]) &&
+                    axis_build(hs.ny - 1, hs.dy, false, true, true, hk, axes[1]) &&
+                    axis_build(hs.nzh, hs.dz, true, true, false, hk, axes[2]);
+    if (!ok) {
+        set_error("heat: tridiagonal eigen-solver did not converge");
+        return nullptr;
+    }
+    HeatGeom* g = new HeatGeom();
+    const double** kp[3] = {&g->ax.kx, &g->ax.ky, &g->ax.kz};
+    const double** wp[3] = {&g->ax.wx, &g->ax.wy, &g->ax.wz};
+    const double** lp[3] = {&g->ax.lx, &g->ax.ly, &g->ax.lz};
+    bool fail = false;
+    for (int i = 0; i < 3 && !fail; i++) {
+        const int n = axes[i].n;
+        double* d = nullptr;
+        fail |= hipMalloc(reinterpret_cast<void**>(&d), 8 * 3 * (size_t)n) != hipSuccess;
+        if (fail) break;
+        g->mem[3 * i] = d;
+        fail |= hipMemcpy(d, axes[i].k.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess;
+        fail |= hipMemcpy(d + n, axes[i].w.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess;
+        fail |= hipMemcpy(d + 2 * n, axes[i].lam.data(), 8 * n, hipMemcpyHostToDevice) != hipSuccess;
+        *kp[i] = d;
+        *wp[i] = d + n;
+        *lp[i] = d + 2 * n;
+        double* q64 = nullptr;
+        float* q32 = nullptr;
+        fail |= hipMalloc(reinterpret_cast<void**>(&q64), 8 * (size_t)n * n) != hipSuccess;
+        if (fail) break;
+        g->mem[3 * i + 1] = q64;
+        fail |= hipMalloc(reinterpret_cast<void**>(&q32), 4 * (size_t)n * n) != hipSuccess;
+        if (fail) break;
+        g->mem[3 * i + 2] = q32;
+        std::vector<float> qf(axes[i].q.begin(), axes[i].q.end());
+        fail |= hipMemcpy(q64, axes[i].q.data(), 8 * (size_t)n * n, hipMemcpyHostToDevice) != hipSuccess;
+        fail |= hipMemcpy(q32, qf.data(), 4 * (size_t)n * n, hipMemcpyHostToDevice) != hipSuccess;
+        g->q64[i] = q64;
+        g->q32[i] = q32;
+    }
+    if (fail) {
+        for (void* m : g->mem)
+            if (m) (void)hipFree(m);
+        delete g;
+        set_error("heat: device allocation of solver tables failed");
+        return nullptr;
+    }
+    h->heat_cache[key] = g;
+    return g;
+}
+
+void heat_cache_free(shg_handle* h) {
+    for (auto& kv : h->heat_cache) {
+        for (void* m : kv.second->mem)
+            if (m) (void)hipFree(m);
+        delete kv.second;
+    }
+    h->heat_cache.clear();
+}
+
 static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t heat_bytes(const shg_params* p) {
@@ -168,14 +254,12 @@ size_t heat_bytes(const shg_params* p) {
     const size_t n3 = (size_t)p->nx * p->ny * nzh;
     const size_t nu = (size_t)p->nx * (p->ny - 1) * nzh;
     const size_t rs = p->precision == SHG_PREC_FP64_PLAIN ? 8 : 4;
-    size_t b = 4 * al256(8 * n3) + 2 * al256(4 * n3);  // T, Tn, dk, dkbase; T32, Q
+    size_t b = 4 * al256(8 * n3) + al256(8 * n3) + al256(4 * n3);  // T, Tn, dk, dkbase; nn, Q
     b += 2 * al256(rs * nu);                           // u, v
     const size_t nb = (size_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * nzh;
     b += al256(8 * nb) + al256(8 * 8);                 // partials, slots
     b += al256(8 * (size_t)p->nz);                     // poling
-    const size_t n[3] = {(size_t)p->nx, (size_t)p->ny - 1, nzh};
-    for (int i = 0; i < 3; i++) b += 3 * al256(8 * n[i]) + al256(rs * n[i] * n[i]);
-    b += al256(sizeof(DirectTables)) + 4096;
+    b += 4096;
     return b;
 }
 
@@ -204,7 +288,7 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     hs.Tn = A.carve<double>(n3);
     hs.dk = A.carve<double>(n3);
     hs.dkbase = A.carve<double>(n3);
-    hs.T32 = A.carve<float>(n3);
+    hs.nn = A.carve<float2>(n3);
     hs.Q = A.carve<float>(n3);
     const HeatCoef c = heat_coef(p, hs.nzh);
     hs.nblocks = heat_grid_blocks(c);
@@ -225,45 +309,16 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
         set_error("arena exhausted (heat buffers)");
         return SHG_ENOMEM;
     }
-    // axes: x (Robin both ends), y' = y-1 (Dirichlet below, Robin top), z (Robin both)
-    const double hk = p->h_conv / p->k_th;
-    AxisHost axes[3];
-    const bool ok = axis_build(hs.nx, hs.dx, true, true, false, hk, axes[0]) &&
-                    axis_build(hs.ny - 1, hs.dy, false, true, true, hk, axes[1]) &&
-                    axis_build(hs.nzh, hs.dz, true, true, false, hk, axes[2]);
-    if (!ok) {
-        set_error("heat: tridiagonal eigen-solver did not converge");
-        return SHG_ENOCONV;
-    }
-    const double** kp[3] = {&dt->ax.kx, &dt->ax.ky, &dt->ax.kz};
-    const double** wp[3] = {&dt->ax.wx, &dt->ax.wy, &dt->ax.wz};
-    const double** lp[3] = {&dt->ax.lx, &dt->ax.ly, &dt->ax.lz};
+    HeatGeom* g = heat_geom(h, p, hs);
+    if (!g) return SHG_ENOCONV;
+    dt->ax = g->ax;
     for (int i = 0; i < 3; i++) {
-        const int n = axes[i].n;
-        double* dk_ = A.carve<double>(n);
-        double* dw = A.carve<double>(n);
-        double* dl = A.carve<double>(n);
-        if (!dk_ || !dw || !dl) {
-            set_error("arena exhausted (heat tables)");
-            return SHG_ENOMEM;
-        }
-        SHG_HIP(hipMemcpy(dk_, axes[i].k.data(), 8 * n, hipMemcpyHostToDevice));
-        SHG_HIP(hipMemcpy(dw, axes[i].w.data(), 8 * n, hipMemcpyHostToDevice));
-        SHG_HIP(hipMemcpy(dl, axes[i].lam.data(), 8 * n, hipMemcpyHostToDevice));
-        *kp[i] = dk_;
-        *wp[i] = dw;
-        *lp[i] = dl;
         if (f64) {
-            double* q = A.carve<double>((size_t)n * n);
-            if (!q) return SHG_ENOMEM;
-            SHG_HIP(hipMemcpy(q, axes[i].q.data(), 8 * (size_t)n * n, hipMemcpyHostToDevice));
-            dt->q[i] = q;
+            if (!g->q64[i]) return SHG_ENOMEM;
+            dt->q[i] = g->q64[i];
         } else {
-            std::vector<float> qf(axes[i].q.begin(), axes[i].q.end());
-            float* q = A.carve<float>((size_t)n * n);
-            if (!q) return SHG_ENOMEM;
-            SHG_HIP(hipMemcpy(q, qf.data(), 4 * (size_t)n * n, hipMemcpyHostToDevice));
-            dt->q[i] = q;
+            if (!g->q32[i]) return SHG_ENOMEM;
+            dt->q[i] = g->q32[i];
         }
     }
     return SHG_OK;
@@ -322,25 +377,27 @@ static int axis_gemm(shg_handle* h, int axis, bool fwd, const R* q, const R* in,
     return SHG_OK;
 }
 
-// one defect-correction step: u = W^-1/2 r; T += W^-1/2 S^-1 u
+// residual pass: u = W^-1/2 A_ii (F(T) - T); optionally returns ||F(T) - T||_2^2
 template <typename R>
-static int direct_step(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* r0_sq,
-                       double* max_corr) {
+static int resid_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* r_sq) {
+    {
+        ProfScope ps(h, "heat_resid");
+        SHG_HIP(launch_heat_resid<R>(h->stream, c, dt.ax, hs.T, hs.Q, static_cast<R*>(dt.u), hs.part));
+    }
+    if (!r_sq) return SHG_OK;
+    SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, false, hs.red + 0));
+    return read_scalar(h, hs.red + 0, r_sq);
+}
+
+// correction pass: T += W^-1/2 S^-1 u (u from the residual pass); returns max|update|
+template <typename R>
+static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* max_corr) {
     const int nx = hs.nx, nyu = hs.ny - 1, nz = hs.nzh;
     R* u = static_cast<R*>(dt.u);
     R* v = static_cast<R*>(dt.v);
     const R* qx = static_cast<const R*>(dt.q[0]);
     const R* qy = static_cast<const R*>(dt.q[1]);
     const R* qz = static_cast<const R*>(dt.q[2]);
-    {
-        ProfScope ps(h, "heat_resid");
-        SHG_HIP(launch_heat_resid<R>(h->stream, c, dt.ax, hs.T, hs.Q, u, hs.part));
-        SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, false, hs.red + 0));
-    }
-    if (r0_sq) {
-        if (int rc = read_scalar(h, hs.red + 0, r0_sq)) return rc;
-        if (!max_corr) return SHG_OK;
-    }
     {
         ProfScope ps(h, "heat_fd_gemm");
         if (int rc = axis_gemm<R>(h, 0, true, qx, u, v, nx, nyu, nz)) return rc;
@@ -357,6 +414,50 @@ static int direct_step(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTa
         SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, true, hs.red + 1));
     }
     return read_scalar(h, hs.red + 1, max_corr);
+}
+
+template <typename R>
+static int direct_solve(shg_handle* h, const shg_params* p, const HeatCoef& c, HeatState& hs, DirectTables& dt,
+                        HeatSolveInfo& info) {
+    // the reference's first check: the change of one sweep from the current T
+    // (its residual pass also feeds the first correction)
+    double r0sq = 0;
+    if (int rc = resid_pass<R>(h, c, hs, dt, &r0sq)) return rc;
+    if (std::sqrt(r0sq) < p->jacobi_tol) {
+        ProfScope ps(h, "heat_jacobi_sweep");
+        SHG_HIP(launch_heat_sweep(h->stream, c, hs.T, hs.Tn, hs.Q, hs.part, false));
+        std::swap(hs.T, hs.Tn);
+        info.sweeps = 1;
+        info.residual = std::sqrt(r0sq);
+        info.converged_in_one = true;
+        return SHG_OK;
+    }
+    // defect correction until the update itself is below tol (each step
+    // contracts the error by ~1e-6 with FP32 transforms, ~1e-14 with FP64)
+    const double tol = p->heat_tol_K > 0 ? p->heat_tol_K : 1e-8;
+    double mc = INFINITY, prev = INFINITY;
+    int it = 0;
+    for (;; it++) {
+        if (it > 0)
+            if (int rc = resid_pass<R>(h, c, hs, dt, nullptr)) return rc;
+        if (int rc = correct_pass<R>(h, c, hs, dt, &mc)) return rc;
+        if (!std::isfinite(mc)) {
+            set_error("heat: non-finite correction");
+            return SHG_ENAN;
+        }
+        if (it > 0 && prev > 0 && std::isfinite(prev)) hs.rho = std::max(mc / prev, 1e-16);
+        prev = mc;
+        if (mc < tol) break;
+        if (it >= 40) {
+            set_error("heat: defect correction did not reach %g K (last update %g K)", tol, mc);
+            return SHG_ENOCONV;
+        }
+    }
+    info.sweeps = 0;
+    info.residual = mc;
+    info.converged_in_one = false;
+    info.refinements = it + 1;
+    return SHG_OK;
 }
 
 int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo& info) {
@@ -395,52 +496,14 @@ int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo&
         return SHG_OK;
     }
     DirectTables& dt = *static_cast<DirectTables*>(hs.solver);
-    // the reference's first check: the change of one sweep from the current T
-    double r0sq = 0;
-    int rc = dt.fp64 ? direct_step<double>(h, c, hs, dt, &r0sq, nullptr)
-                     : direct_step<float>(h, c, hs, dt, &r0sq, nullptr);
-    if (rc) return rc;
-    if (std::sqrt(r0sq) < p->jacobi_tol) {
-        ProfScope ps(h, "heat_jacobi_sweep");
-        SHG_HIP(launch_heat_sweep(h->stream, c, hs.T, hs.Tn, hs.Q, hs.part, false));
-        std::swap(hs.T, hs.Tn);
-        info.sweeps = 1;
-        info.residual = std::sqrt(r0sq);
-        info.converged_in_one = true;
-        return SHG_OK;
-    }
-    const double tol = p->heat_tol_K > 0 ? p->heat_tol_K : 1e-8;
-    double mc = INFINITY;
-    int it = 0;
-    for (; it < 40; it++) {
-        rc = dt.fp64 ? direct_step<double>(h, c, hs, dt, nullptr, &mc) : direct_step<float>(h, c, hs, dt, nullptr, &mc);
-        if (rc) return rc;
-        if (!std::isfinite(mc)) {
-            set_error("heat: non-finite correction");
-            return SHG_ENAN;
-        }
-        if (mc < tol) break;
-    }
-    if (mc >= tol) {
-        set_error("heat: defect correction did not reach %g K (last update %g K)", tol, mc);
-        return SHG_ENOCONV;
-    }
-    // report the reference's measure (one-sweep change) at the solution
-    double rs = 0;
-    rc = dt.fp64 ? direct_step<double>(h, c, hs, dt, &rs, nullptr) : direct_step<float>(h, c, hs, dt, &rs, nullptr);
-    if (rc) return rc;
-    info.sweeps = 0;
-    info.residual = std::sqrt(rs);
-    info.converged_in_one = false;
-    info.refinements = it + 1;
-    return SHG_OK;
+    return dt.fp64 ? direct_solve<double>(h, p, c, hs, dt, info) : direct_solve<float>(h, p, c, hs, dt, info);
 }
 
 int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc) {
     const long n3 = (long)hs.nx * hs.ny * hs.nzh;
     {
         ProfScope ps(h, "heat_dk");
-        SHG_HIP(launch_heat_dk(h->stream, hs.T, hs.dk, hs.T32, n3, p->crystal, p->lambda_p_um, p->Lambda_um,
+        SHG_HIP(launch_heat_dk(h->stream, hs.T, hs.dk, hs.nn, n3, p->crystal, p->lambda_p_um, p->Lambda_um,
                                p->therm_exp_a, p->therm_exp_b));
         SHG_HIP(launch_heat_dkbase(h->stream, hs.dk, hs.dkbase, hs.nx * hs.ny, hs.nzh, p->nz));
     }

@@ -15,7 +15,7 @@ struct HeatState {
     double dx = 0, dy = 0, dz = 0;
     double* T = nullptr;       // FP64 temperature [nzh][ny][nx]
     double* Tn = nullptr;      // FP64 scratch
-    float* T32 = nullptr;      // FP32 copy read by the optics
+    float2* nn = nullptr;      // (n_p(T), n_s(T)) per voxel, read by the optics
     float* Q = nullptr;        // heat source [nzh][ny][nx]
     double* dk = nullptr;      // dk(T) [nzh][ny][nx]
     double* dkbase = nullptr;  // running-integral base per heat slice
@@ -24,6 +24,7 @@ struct HeatState {
     double* part = nullptr;    // per-block partials
     int nblocks = 0;
     void* solver = nullptr;    // direct-solver tables (opaque)
+    double rho = 0.0;          // observed contraction of one defect-correction step
     bool converged_in_one = false;
 };
 

@@ -9,9 +9,10 @@ namespace shg {
 struct HeatCoef {
     int nx, ny, nz;
     double kth, Tinf;
-    double idx2, idy2, idz2, L2;
+    double idx2, idy2, idz2, L2, ikth;
     double Bix, Biy, Biz;
     double ax, ay, az, bx, by, bz;
+    double Fx, Fy, Fz;  // 1/(1 + 2a + 2b + Bi) of the x, top and z faces
     double av, bv, cv, V, ah, bh, ch, H, ad, bd, cd, D;
     double AV, BV, CV, BiD, GV;
 };
@@ -41,7 +42,7 @@ hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a
                             double* part_max);
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out);
 hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel);
-hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float* T32, long n, int crystal, double lp,
+hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, long n, int crystal, double lp,
                           double Lambda, double ea, double eb);
 hipError_t launch_heat_dkbase(hipStream_t st, const double* dk, double* base, int n2, int nzh, int nz);
 hipError_t launch_max(hipStream_t st, const double* a, long n, double* part, int nblocks);

@@ -17,7 +17,7 @@ struct MixedInst {
     const double2* eyP;     // y factor [ny]
     const double2* eyS;
     const float2* phase;    // non-THERMAL: exp(i phi) [nz][2 calls][3 offsets]
-    const float* T;         // THERMAL: temperature [nzh][ny][nx]
+    const float2* nn;       // THERMAL: (n_p(T), n_s(T)) per heat voxel [nzh][ny][nx]
     const double* dkbase;   // THERMAL: running dk integral = dkbase[h] + s*dk[h]
     const double* dk;       // THERMAL: dk(T) [nzh][ny][nx]
     float* Q;               // THERMAL: heat source [nzh][ny][nx]
@@ -37,9 +37,9 @@ struct MixedInst {
 
 // FP32-mixed pipeline (kernels_mixed.hip)
 hipError_t launch_rowpass(int nx, bool thermal, int rows_pw, dim3 grid, hipStream_t st,
-                          const MixedInst* insts, int g, const float4* tw);
+                          const MixedInst* insts, int g, const double2* tw);
 hipError_t launch_colpass(int ny, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
-                          int g, const float4* tw);
+                          int g, const double2* tw);
 int rowpass_rows_default(int nx);
 int colpass_cols_default(int ny);
 bool colpass_cols_supported(int ny, int cols);

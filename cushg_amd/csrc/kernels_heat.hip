@@ -1,125 +1,143 @@
 // kernels_heat.hip -- THERMAL kernels: the reference's Jacobi sweep (as a
 // sweep and as an FP64 residual), the pointwise stages of the
-// fast-diagonalisation solve, dk(T) and the running dk integral base, and
-// the heat source clear.
+// fast-diagonalisation solve, dk(T) and the running dk integral base.
+//
+// The stencil kernels march along z inside each thread (threads cover the
+// x-y plane, x fastest), keeping T at z-1, z, z+1 in registers: every
+// voxel's temperature is read from memory once per sweep (plus its x/y
+// neighbours, which hit the caches), as in the reference's in-thread
+// z-loop (src/headers/Tfield.h:111).
 #include "heat_kernels.hpp"
 
 namespace shg {
 
-// ------------------------------------------------------------------
-// One application of the Jacobi map of src/headers/Tfield.h:93-201 at
-// node (x, y, z), y >= 1 (the y = 0 plane is the Dirichlet oven).
-// ------------------------------------------------------------------
-__device__ __forceinline__ double jacobi_node(const HeatCoef& c, const double* __restrict__ Ti,
-                                              const float* __restrict__ Q, int x, int y, int z) {
-    const int NX = c.nx, NY = c.ny, NZ = c.nz;
-    const size_t sx = 1, sy = (size_t)NX, sz = (size_t)NX * NY;
-    const size_t i = (size_t)z * sz + (size_t)y * sy + x;
-    const bool xi = x > 0 && x < NX - 1, yi = y < NY - 1, zi = z > 0 && z < NZ - 1;
+struct Nb {
+    double c, xm, xp, ym, yp, zm, zp;
+};
+
+// One application of the Jacobi map of src/headers/Tfield.h:93-201 at node
+// (x, y, z), y >= 1 (the y = 0 plane is the Dirichlet oven), given the
+// values of its neighbours (those outside the crystal are not used).
+__device__ __forceinline__ double jacobi_vals(const HeatCoef& c, int x, int y, int z, const Nb& t, float q) {
+    const bool xi = x > 0 && x < c.nx - 1, yi = y < c.ny - 1, zi = z > 0 && z < c.nz - 1;
     const bool xe = !xi, ze = !zi;
-    const size_t ixn = x == 0 ? i + sx : i - sx;  // inward x neighbour
-    const size_t izn = z == 0 ? i + sz : i - sz;  // inward z neighbour
+    const double xn = x == 0 ? t.xp : t.xm;  // inward x neighbour
+    const double zn = z == 0 ? t.zp : t.zm;  // inward z neighbour
     const double Tinf = c.Tinf;
     if (xi && yi && zi)  // inner points, :113-115
-        return c.L2 * ((Ti[i + sx] + Ti[i - sx]) * c.idx2 + (Ti[i + sy] + Ti[i - sy]) * c.idy2 +
-                       (Ti[i + sz] + Ti[i - sz]) * c.idz2 + (double)Q[i] / c.kth);
+        return c.L2 * ((t.xp + t.xm) * c.idx2 + (t.yp + t.ym) * c.idy2 + (t.zp + t.zm) * c.idz2 + (double)q * c.ikth);
     if (xe && yi && zi)  // left/right faces, :119-125
-        return (Ti[ixn] + c.ax * (Ti[i + sy] + Ti[i - sy]) + c.bx * (Ti[i + sz] + Ti[i - sz]) + c.Bix * Tinf) /
-               (1 + 2 * c.ax + 2 * c.bx + c.Bix);
+        return (xn + c.ax * (t.yp + t.ym) + c.bx * (t.zp + t.zm) + c.Bix * Tinf) * c.Fx;
     if (xi && !yi && zi)  // top face, :127-129
-        return (Ti[i - sy] + c.ay * (Ti[i + sx] + Ti[i - sx]) + c.by * (Ti[i + sz] + Ti[i - sz]) + c.Biy * Tinf) /
-               (1 + 2 * c.ay + 2 * c.by + c.Biy);
+        return (t.ym + c.ay * (t.xp + t.xm) + c.by * (t.zp + t.zm) + c.Biy * Tinf) * c.Fy;
     if (xi && yi && ze)  // back/front faces, :131-137
-        return (Ti[izn] + c.az * (Ti[i + sx] + Ti[i - sx]) + c.bz * (Ti[i + sy] + Ti[i - sy]) + c.Biz * Tinf) /
-               (1 + 2 * c.az + 2 * c.bz + c.Biz);
+        return (zn + c.az * (t.xp + t.xm) + c.bz * (t.yp + t.ym) + c.Biz * Tinf) * c.Fz;
     if (xe && yi && ze)  // vertical sides, :143-157
-        return (c.av * Ti[ixn] + c.bv * Ti[izn] + c.cv * (Ti[i + sy] + Ti[i - sy]) + 0.5 * Tinf * (c.Bix + c.Biz)) * c.V;
+        return (c.av * xn + c.bv * zn + c.cv * (t.yp + t.ym) + 0.5 * Tinf * (c.Bix + c.Biz)) * c.V;
     if (xi && !yi && ze)  // horizontal sides, :161-167
-        return (c.ah * Ti[i - sy] + c.bh * Ti[izn] + c.ch * (Ti[i - sx] + Ti[i + sx]) + 0.5 * Tinf * (c.Biy + c.Biz)) *
-               c.H;
+        return (c.ah * t.ym + c.bh * zn + c.ch * (t.xm + t.xp) + 0.5 * Tinf * (c.Biy + c.Biz)) * c.H;
     if (xe && !yi && zi)  // depth sides, :171-177
-        return (c.ad * Ti[i - sy] + c.bd * Ti[ixn] + c.cd * (Ti[i + sz] + Ti[i - sz]) + 0.5 * Tinf * (c.Bix + c.Biy)) *
-               c.D;
+        return (c.ad * t.ym + c.bd * xn + c.cd * (t.zp + t.zm) + 0.5 * Tinf * (c.Bix + c.Biy)) * c.D;
     // vertices, :181-196
-    return (c.AV * Ti[ixn] + c.BV * Ti[i - sy] + c.CV * Ti[izn] + Tinf * c.BiD) * c.GV;
+    return (c.AV * xn + c.BV * t.ym + c.CV * zn + Tinf * c.BiD) * c.GV;
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red) {
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    const int n = blockDim.x * blockDim.y;
-    red[t] = v;
-    __syncthreads();
-    for (int s = n / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] += red[t + s];
-        __syncthreads();
+// Walks the z column of node (x, y), y >= 1, calling f(z, index, F(T), F(T) - T).
+template <typename Fn>
+__device__ __forceinline__ void jacobi_column(const HeatCoef& c, const double* __restrict__ T,
+                                              const float* __restrict__ Q, int x, int y, Fn&& f) {
+    const size_t sy = (size_t)c.nx, sz = (size_t)c.nx * c.ny;
+    size_t i = (size_t)y * sy + x;
+    const bool hx0 = x > 0, hx1 = x < c.nx - 1, hy1 = y < c.ny - 1;
+    double cur = T[i], nxt = c.nz > 1 ? T[i + sz] : 0.0, prv = 0.0;
+    for (int z = 0; z < c.nz; z++, i += sz) {
+        Nb t;
+        t.c = cur;
+        t.zm = prv;
+        t.zp = nxt;
+        t.xm = hx0 ? T[i - 1] : 0.0;
+        t.xp = hx1 ? T[i + 1] : 0.0;
+        t.ym = T[i - sy];
+        t.yp = hy1 ? T[i + sy] : 0.0;
+        const float q = Q[i];
+        const double v = jacobi_vals(c, x, y, z, t, q);
+        f(z, i, v, v - cur);
+        prv = cur;
+        cur = nxt;
+        nxt = z + 2 < c.nz ? T[i + 2 * sz] : 0.0;
     }
-    const double r = red[0];
-    __syncthreads();
-    return r;
 }
-__device__ __forceinline__ double block_max(double v, double* red) {
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    const int n = blockDim.x * blockDim.y;
-    red[t] = v;
-    __syncthreads();
-    for (int s = n / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] = fmax(red[t], red[t + s]);
-        __syncthreads();
+
+// block (64, 4) reductions with wave shuffles
+__device__ __forceinline__ double block_reduce(double v, bool is_max, double* red) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = is_max ? fmax(v, o) : v + o;
     }
-    const double r = red[0];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();
+    double r = 0.0;
+    if (t == 0) {
+        r = red[0];
+        const int nw = (blockDim.x * blockDim.y + 63) / 64;
+        for (int w = 1; w < nw; w++) r = is_max ? fmax(r, red[w]) : r + red[w];
+    }
     return r;
 }
 
 // Jacobi sweep Tf = F(Ti) (Tfield::upDate); with `want_norm`, per-block
 // partial sums of (Tf - Ti)^2 (Tfield::checkConvergence, :366-376).
-// block (64, 4), grid (nx/64, ny/4, nz)
+// block (64, 4), grid (nx/64, ny/4)
 __global__ __launch_bounds__(256) void k_heat_sweep(HeatCoef c, const double* __restrict__ Ti,
                                                     double* __restrict__ Tf, const float* __restrict__ Q,
                                                     double* __restrict__ part_sq, int want_norm) {
-    __shared__ double red[256];
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    __shared__ double red[4];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     double d2 = 0.0;
     if (x < c.nx && y < c.ny) {
-        const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
         if (y == 0) {
-            Tf[i] = Ti[i];
+            for (int z = 0; z < c.nz; z++) {
+                const size_t i = ((size_t)z * c.ny) * c.nx + x;
+                Tf[i] = Ti[i];
+            }
         } else {
-            const double v = jacobi_node(c, Ti, Q, x, y, z);
-            d2 = (v - Ti[i]) * (v - Ti[i]);
-            Tf[i] = v;
+            jacobi_column(c, Ti, Q, x, y, [&](int, size_t i, double v, double d) {
+                Tf[i] = v;
+                d2 += d * d;
+            });
         }
     }
     if (want_norm) {
-        const double s = block_sum(d2, red);
-        if (threadIdx.x == 0 && threadIdx.y == 0)
-            part_sq[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+        const double s = block_reduce(d2, false, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0) part_sq[blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
 }
 
 // FP64 residual of the steady state: delta = F(T) - T (the reference's
 // one-sweep change) and the finite-volume residual r = A_ii * delta, written
 // scaled by W^-1/2 into the unknown layout u[z][y-1][x] that the
-// fast-diagonalisation transforms consume.  Partial sums of delta^2.
+// fast-diagonalisation transforms consume; partial sums of delta^2.
 template <typename R>
 __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, const double* __restrict__ T,
                                                     const float* __restrict__ Q, R* __restrict__ u,
                                                     double* __restrict__ part_sq) {
-    __shared__ double red[256];
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    __shared__ double red[4];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     double d2 = 0.0;
     if (x < c.nx && y >= 1 && y < c.ny) {
-        const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
-        const double d = jacobi_node(c, T, Q, x, y, z) - T[i];
-        d2 = d * d;
         const int j = y - 1;
-        const double wx = a.wx[x], wy = a.wy[j], wz = a.wz[z];
-        const double aii = a.kx[x] * wy * wz + wx * a.ky[j] * wz + wx * wy * a.kz[z];
-        u[((size_t)z * (c.ny - 1) + j) * c.nx + x] = (R)(aii * d * rsqrt(wx * wy * wz));
+        const double wx = a.wx[x], wy = a.wy[j], kx = a.kx[x], ky = a.ky[j];
+        const size_t nyu = (size_t)(c.ny - 1);
+        jacobi_column(c, T, Q, x, y, [&](int z, size_t, double, double d) {
+            d2 += d * d;
+            const double wz = a.wz[z];
+            const double aii = kx * wy * wz + wx * ky * wz + wx * wy * a.kz[z];
+            u[((size_t)z * nyu + j) * c.nx + x] = (R)(aii * d * rsqrt(wx * wy * wz));
+        });
     }
-    const double s = block_sum(d2, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-        part_sq[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    const double s = block_reduce(d2, false, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part_sq[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
 // spectral division by lambda_x + lambda_y + lambda_z
@@ -136,22 +154,25 @@ __global__ void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx, co
     }
 }
 
-// T += W^-1/2 u (the correction), partial max |correction|
+// T += W^-1/2 u (the correction); partial max |correction|
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_update(HeatCoef c, HeatAxes a, const R* __restrict__ u,
                                                    double* __restrict__ T, double* __restrict__ part_max) {
-    __shared__ double red[256];
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    __shared__ double red[4];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     double m = 0.0;
     if (x < c.nx && y >= 1 && y < c.ny) {
         const int j = y - 1;
-        const double corr = (double)u[((size_t)z * (c.ny - 1) + j) * c.nx + x] * rsqrt(a.wx[x] * a.wy[j] * a.wz[z]);
-        T[((size_t)z * c.ny + y) * c.nx + x] += corr;
-        m = fabs(corr);
+        const double wxy = a.wx[x] * a.wy[j];
+        const size_t nyu = (size_t)(c.ny - 1);
+        for (int z = 0; z < c.nz; z++) {
+            const double corr = (double)u[((size_t)z * nyu + j) * c.nx + x] * rsqrt(wxy * a.wz[z]);
+            T[((size_t)z * c.ny + y) * c.nx + x] += corr;
+            m = fmax(m, fabs(corr));
+        }
     }
-    const double s = block_max(m, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-        part_max[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    const double s = block_reduce(m, true, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part_max[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part, int n, int op,
@@ -182,13 +203,19 @@ __global__ void k_heat_init(double* __restrict__ T, float* __restrict__ Q, int n
 }
 
 // dk(T) per voxel in FP64 (PhaseMatching::setDKFromTemperature,
-// src/headers/PhaseMatching.h:47-65) and the FP32 copy of T for the optics
-__global__ void k_heat_dk(const double* __restrict__ T, double* __restrict__ dk, float* __restrict__ T32, long n,
+// src/headers/PhaseMatching.h:47-65) and (n_p(T), n_s(T)) per voxel for the
+// optics (the n(lambda, T) of src/headers/Solver.h:48,50 and
+// src/headers/Tfield.h:270), from one FP64 Sellmeier pair per voxel
+__global__ void k_heat_dk(const double* __restrict__ T, double* __restrict__ dk, float2* __restrict__ nn, long n,
                           int crystal, double lp, double Lambda, double ea, double eb) {
+    const double ls = 0.5 * lp;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
         const double t = T[i];
-        dk[i] = delta_k(crystal, lp, Lambda, ea, eb, t);
-        T32[i] = (float)t;
+        const double np = refr_index<double>(crystal, lp, t);
+        const double ns = refr_index<double>(crystal, ls, t);
+        const double d = t - 25.0;
+        dk[i] = 2.0 * kPi * (2.0 * np / lp - ns / ls + 1.0 / (Lambda * (1.0 + ea * d + eb * d * d)));
+        nn[i] = make_float2((float)np, (float)ns);
     }
 }
 
@@ -224,22 +251,22 @@ __global__ __launch_bounds__(256) void k_max(const double* __restrict__ a, long 
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
-static dim3 heat_grid(const HeatCoef& c) { return dim3((c.nx + 63) / 64, (c.ny + 3) / 4, c.nz); }
+static dim3 plane_grid(const HeatCoef& c) { return dim3((c.nx + 63) / 64, (c.ny + 3) / 4); }
 int heat_grid_blocks(const HeatCoef& c) {
-    const dim3 g = heat_grid(c);
-    return (int)(g.x * g.y * g.z);
+    const dim3 g = plane_grid(c);
+    return (int)(g.x * g.y);
 }
 
 hipError_t launch_heat_sweep(hipStream_t st, const HeatCoef& c, const double* Ti, double* Tf, const float* Q,
                              double* part_sq, bool want_norm) {
-    hipLaunchKernelGGL(k_heat_sweep, heat_grid(c), dim3(64, 4), 0, st, c, Ti, Tf, Q, part_sq, want_norm ? 1 : 0);
+    hipLaunchKernelGGL(k_heat_sweep, plane_grid(c), dim3(64, 4), 0, st, c, Ti, Tf, Q, part_sq, want_norm ? 1 : 0);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* T, const float* Q,
                              R* u, double* part_sq) {
-    hipLaunchKernelGGL((k_heat_resid<R>), heat_grid(c), dim3(64, 4), 0, st, c, a, T, Q, u, part_sq);
+    hipLaunchKernelGGL((k_heat_resid<R>), plane_grid(c), dim3(64, 4), 0, st, c, a, T, Q, u, part_sq);
     return hipGetLastError();
 }
 template hipError_t launch_heat_resid<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*,
@@ -261,7 +288,7 @@ template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxe
 template <typename R>
 hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
                             double* part_max) {
-    hipLaunchKernelGGL((k_fd_update<R>), heat_grid(c), dim3(64, 4), 0, st, c, a, u, T, part_max);
+    hipLaunchKernelGGL((k_fd_update<R>), plane_grid(c), dim3(64, 4), 0, st, c, a, u, T, part_max);
     return hipGetLastError();
 }
 template hipError_t launch_fd_update<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const float*, double*,
@@ -279,9 +306,9 @@ hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny,
     return hipGetLastError();
 }
 
-hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float* T32, long n, int crystal, double lp,
+hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, long n, int crystal, double lp,
                           double Lambda, double ea, double eb) {
-    hipLaunchKernelGGL(k_heat_dk, dim3(2048), dim3(256), 0, st, T, dk, T32, n, crystal, lp, Lambda, ea, eb);
+    hipLaunchKernelGGL(k_heat_dk, dim3(4096), dim3(256), 0, st, T, dk, nn, n, crystal, lp, Lambda, ea, eb);
     return hipGetLastError();
 }
 

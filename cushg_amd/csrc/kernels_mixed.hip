@@ -75,29 +75,34 @@ __device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const Mix
         const size_t N2 = (size_t)nx * I.ny;
         const int h = heat_slice(step, I.nz, I.nzh);
         const size_t rowoff = (size_t)h * N2 + (size_t)y * nx;
-        const float* Tr = I.T + rowoff;
+        const float2* nr = I.nn + rowoff;
         const double* dkb = I.dkbase + rowoff;
         const double* dkr = I.dk + rowoff;
         const double pol = I.poling ? I.poling[step] : 0.0;
         const double zsc = I.dz / ((double)step + 1.0);
         const double o0 = (double)step + I.off[call][0];
-        const double o1 = (double)step + I.off[call][1];
-        const double o2 = (double)step + I.off[call][2];
         float* Qr = I.Q + rowoff;
         const bool first = step == 0 || heat_slice(step - 1, I.nz, I.nzh) != h;
         const float qw = 1.0f / (float)heat_count(h, I.nz, I.nzh);
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int x = tp + q * TPR;
-            const float T = Tr[x];
-            const float np = refr_index<float>(I.crystal, I.lp, T);
-            const float ns = refr_index<float>(I.crystal, I.ls, T);
-            const float kp = I.kp0 / np, ks = I.ks0 / ns, bfac = I.bf0 * ns;
+            const float2 nps = nr[x];  // n_p(T), n_s(T) of this voxel
+            const float np = nps.x, ns = nps.y;
+            // 1/n by the hardware reciprocal refined with one Newton step
+            float rp = __builtin_amdgcn_rcpf(np), rs = __builtin_amdgcn_rcpf(ns);
+            rp = rp * __builtin_fmaf(-np, rp, 2.0f);
+            rs = rs * __builtin_fmaf(-ns, rs, 2.0f);
+            const float kp = I.kp0 * rp, ks = I.ks0 * rs, bfac = I.bf0 * ns;
             // running dk integral (src/headers/PhaseMatching.h:27-44) and the
-            // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50
+            // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50;
+            // the three offsets are 0.25 dz apart in both schemes, so
+            // e(off + 1/4) = e(off) exp(i DKint dz / (4 (s + 1))).
             const double dki = dkb[x] + (double)step * dkr[x];
-            const float2 e[3] = {expi_reduced(dki * o0 * zsc + pol), expi_reduced(dki * o1 * zsc + pol),
-                                 expi_reduced(dki * o2 * zsc + pol)};
+            const float2 e0 = expi_reduced(dki * o0 * zsc + pol);
+            const float2 ed = expi_reduced(0.25 * dki * zsc);
+            const float2 e1 = cmul(e0, ed);
+            const float2 e[3] = {e0, e1, cmul(e1, ed)};
             rk4_point(p[q], s[q], e, kp, ks, aph, ash, bfac, c1, c2, c3, w);
             if (deposit_q) {
                 // Q = a_p I_p + a_s I_s + b_s I_s^2 -- src/headers/Tfield.h:260-276
@@ -117,27 +122,28 @@ __device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const Mix
 // grid (ceil(max_ny/ROWS), 1, n_instances), block (NX/8, ROWS)
 template <int NX, bool THERMAL, int ROWS>
 __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __restrict__ insts, int g,
-                                                           const float4* __restrict__ tw) {
+                                                           const double2* __restrict__ tw) {
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
-    __shared__ float2 lds[ROWS * LD];
+    __shared__ float2 lds[2 * ROWS * LD];
     const MixedInst& I = insts[blockIdx.z];
     const int nz = I.nz;
     if (g > nz) return;
     const int y = blockIdx.x * ROWS + threadIdx.y;
     if (blockIdx.x * ROWS >= I.ny) return;
     const int tp = threadIdx.x;
-    float2* my = lds + threadIdx.y * LD;
+    float2* my = lds + threadIdx.y * LD;  // pump buffer; SH buffer ROWS*LD further
     const size_t base = (size_t)y * NX + tp;
-    float2 p[8], s[8];
+    float2 f[2][8];  // f[0] pump, f[1] SH: transformed together (shared barriers)
+    float2(&p)[8] = f[0];
+    float2(&s)[8] = f[1];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         p[q] = I.P[base + q * TPR];
         s[q] = I.S[base + q * TPR];
     }
     if (g > 0) {
-        fft_dist<float2, NX, true>(p, my, tw, tp);
-        fft_dist<float2, NX, true>(s, my, tw, tp);
+        fft_multi<float2, NX, true, 2>(f, my, ROWS * LD, tw, tp);
         nl_row<NX, THERMAL>(p, s, I, g - 1, 1, y, tp, THERMAL);
         if (g - 1 == nz - 2) {
 #pragma unroll
@@ -149,8 +155,7 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
     }
     if (g < nz) {
         nl_row<NX, THERMAL>(p, s, I, g, 0, y, tp, false);
-        fft_dist<float2, NX, false>(p, my, tw, tp);
-        fft_dist<float2, NX, false>(s, my, tw, tp);
+        fft_multi<float2, NX, false, 2>(f, my, ROWS * LD, tw, tp);
     }
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
 // grid (ceil(max_nx/COLS), 2, n_instances), block (COLS, NY/8)
 template <int NY, int COLS>
 __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __restrict__ insts, int g,
-                                                           const float4* __restrict__ tw) {
+                                                           const double2* __restrict__ tw) {
     constexpr int TPC = NY / 8;
     constexpr int LD = FftShape<NY>::LDS;
     __shared__ float2 lds[COLS * LD];
@@ -202,14 +207,16 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __re
 // dispatch
 // ------------------------------------------------------------------
 
+// Built-in tile plans (measured on MI355X: rows per workgroup of the row
+// pass, columns per workgroup of the column pass); shg_autotune refines them.
 int rowpass_rows_default(int nx) {
     switch (nx) {
-        case 64: return 16;
-        case 128: return 8;
-        case 192: return 4;
-        case 256: return 4;
-        case 384: return 2;
-        case 512: return 2;
+        case 64: return 8;
+        case 128: return 4;
+        case 192: return 2;
+        case 256: return 2;
+        case 384: return 1;
+        case 512: return 1;
     }
     return 0;
 }
@@ -218,10 +225,10 @@ bool rowpass_rows_supported(int nx, int rows) {
 }
 int colpass_cols_default(int ny) {
     switch (ny) {
-        case 64: return 16;
-        case 128: return 16;
-        case 192: return 16;
-        case 256: return 16;
+        case 64: return 8;
+        case 128: return 8;
+        case 192: return 8;
+        case 256: return 8;
         case 384: return 8;
         case 512: return 8;
     }
@@ -233,7 +240,7 @@ bool colpass_cols_supported(int ny, int cols) {
 
 template <int NX, bool TH>
 static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
-                               const float4* tw) {
+                               const double2* tw) {
     dim3 block(NX / 8, rows);
     switch (rows) {
         case 1: hipLaunchKernelGGL((k_rowpass<NX, TH, 1>), grid, block, 0, st, insts, g, tw); break;
@@ -257,7 +264,7 @@ static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedI
 }
 
 hipError_t launch_rowpass(int nx, bool thermal, int rows, dim3 grid, hipStream_t st,
-                          const MixedInst* insts, int g, const float4* tw) {
+                          const MixedInst* insts, int g, const double2* tw) {
 #define ROWCASE(N)                                                                 \
     case N:                                                                        \
         return thermal ? rowpass_rows<N, true>(rows, grid, st, insts, g, tw)       \
@@ -276,7 +283,7 @@ hipError_t launch_rowpass(int nx, bool thermal, int rows, dim3 grid, hipStream_t
 
 template <int NY>
 static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
-                               const float4* tw) {
+                               const double2* tw) {
     dim3 block(cols, NY / 8);
     switch (cols) {
         case 2: hipLaunchKernelGGL((k_colpass<NY, 2>), grid, block, 0, st, insts, g, tw); break;
@@ -294,7 +301,7 @@ static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedI
 }
 
 hipError_t launch_colpass(int ny, int cols, dim3 grid, hipStream_t st, const MixedInst* insts,
-                          int g, const float4* tw) {
+                          int g, const double2* tw) {
     switch (ny) {
         case 64: return colpass_cols<64>(cols, grid, st, insts, g, tw);
         case 128: return colpass_cols<128>(cols, grid, st, insts, g, tw);

@@ -191,7 +191,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         bool thermal;
         std::vector<MixedHost*> m;
         MixedInst* d = nullptr;
-        const float4* tw = nullptr;
+        const double2* tw = nullptr;
     };
     std::map<std::pair<int, bool>, Group> rowg;
     std::map<int, Group> colg;
@@ -218,7 +218,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             return SHG_ENOMEM;
         }
         SHG_HIP(hipMemcpyAsync(g.d, v.data(), sizeof(MixedInst) * v.size(), hipMemcpyHostToDevice, h->stream));
-        g.tw = twiddles32(h, g.key_n);
+        g.tw = twiddles64(h, g.key_n);
         if (!g.tw) {
             set_error("twiddle table allocation failed");
             return SHG_ENOMEM;
@@ -339,7 +339,7 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         d.eyS = m.eyS;
         d.phase = m.phase;
         if (m.thermal) {
-            d.T = m.heat.T32;
+            d.nn = m.heat.nn;
             d.dkbase = m.heat.dkbase;
             d.dk = m.heat.dk;
             d.Q = m.heat.Q;
@@ -387,7 +387,7 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
             m->heat.converged_in_one = info.converged_in_one;
             rc = heat_prepare_optics(h, m->p, m->heat, &m->Tc);
             if (rc) return rc;
-            m->desc.T = m->heat.T32;
+            m->desc.nn = m->heat.nn;
             if ((rc = setup_fields(h, *m))) return rc;
             act.push_back(m);
         }

@@ -94,18 +94,12 @@ static void tw_value(int n, int k, double& c, double& s) {
     c = std::cos(a);
     s = std::sin(a);
 }
-static void tw_set(float4& t, double c, double s) {
-    t.x = (float)c;
-    t.y = (float)s;
-    t.z = (float)(c - (double)t.x);
-    t.w = (float)(s - (double)t.y);
-}
 static void tw_set(double2& t, double c, double s) {
     t.x = c;
     t.y = s;
 }
 
-// tw[m] = exp(-2 pi i m / n) from FP64; FP32 tables hold hi + lo parts
+// tw[m] = exp(-2 pi i m / n) in FP64
 template <typename C>
 static C* make_twiddles(int n) {
     std::vector<C> t(n);
@@ -120,14 +114,6 @@ static C* make_twiddles(int n) {
         (void)hipFree(d);
         return nullptr;
     }
-    return d;
-}
-
-const float4* twiddles32(shg_handle* h, int n) {
-    auto it = h->tw32.find(n);
-    if (it != h->tw32.end()) return it->second;
-    float4* d = make_twiddles<float4>(n);
-    if (d) h->tw32[n] = d;
     return d;
 }
 

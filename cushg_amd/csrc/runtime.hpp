@@ -49,13 +49,17 @@ struct KernelTime {
 
 }  // namespace shg
 
+namespace shg {
+struct HeatGeom;
+}
+
 struct shg_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     shg::Arena arena;
-    std::map<int, float4*> tw32;
     std::map<int, double2*> tw64;
     void* blas = nullptr;         // rocBLAS handle (heat solver), created lazily
+    std::map<std::string, shg::HeatGeom*> heat_cache;  // direct-solver tables per geometry
     double* d_scalars = nullptr;  // reduction slots
     double* d_partials = nullptr;
     int n_partials = 0;
@@ -83,7 +87,6 @@ struct ProfScope {
 void prof_collect(shg_handle* h);
 void prof_reset(shg_handle* h);
 
-const float4* twiddles32(shg_handle* h, int n);
 const double2* twiddles64(shg_handle* h, int n);
 
 int sum_sq(shg_handle* h, const float2* A, long n, double* slot);

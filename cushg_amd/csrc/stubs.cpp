@@ -2,7 +2,6 @@
 #include "heat.hpp"
 #include "runtime.hpp"
 namespace shg {
-int run_plain(shg_handle*, const shg_params*, shg_result*) { set_error("FP64 plain not built"); return SHG_EUNSUPPORTED; }
 int run_adjoint(shg_handle*, const shg_params*, int, double*, shg_result*) { set_error("adjoint not built"); return SHG_EUNSUPPORTED; }
 }
 extern "C" {
