@@ -12,8 +12,9 @@
 
 namespace shg {
 const char* last_error();
-int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs);
-int run_plain(shg_handle* h, const shg_params* p, shg_result* r);
+int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs,
+              SliceTrace* trace = nullptr);
+int run_plain(shg_handle* h, const shg_params* p, shg_result* r, SliceTrace* trace = nullptr);
 int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r);
 void heat_cache_free(shg_handle* h);
 }  // namespace shg
@@ -247,6 +248,33 @@ int shg_run(shg_handle* h, const shg_params* p, shg_result* r) {
         return SHG_EINVAL;
     }
     return shg_run_batch(h, p, 1, r);
+}
+
+int shg_run_traced(shg_handle* h, const shg_params* p, int32_t every, shg_slice_fn fn, void* user,
+                   shg_result* r) {
+    if (!h || !p || !r || !fn) {
+        set_error("shg_run_traced: null argument");
+        return SHG_EINVAL;
+    }
+    clear_error();
+    r->status = shg_validate(p);
+    if (r->status != SHG_OK) return r->status;
+    SHG_HIP(hipSetDevice(h->device));
+    SliceTrace tr;
+    tr.every = every > 0 ? every : 1;
+    tr.fn = fn;
+    tr.user = user;
+    int rc;
+    if (p->precision == SHG_PREC_FP64_PLAIN) {
+        rc = run_plain(h, p, r, &tr);
+    } else {
+        std::vector<const shg_params*> ps{p};
+        std::vector<shg_result*> rs{r};
+        rc = run_mixed(h, ps, rs, &tr);
+        if (rc == SHG_OK) rc = r->status;
+    }
+    r->status = rc;
+    return rc;
 }
 
 int shg_adjoint(shg_handle* h, const shg_params* p, int32_t checkpoint_every, double* d_eta,

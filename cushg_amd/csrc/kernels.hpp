@@ -12,6 +12,9 @@ struct MixedInst {
     float2* S;              // SH field   [ny][nx]
     float2* outP;           // state after nz-1 slices (the reference's Pump_out)
     float2* outS;
+    float2* trP;            // per-slice trace: state before slice g's SSFM step (or null)
+    float2* trS;
+    int trace_every;
     const double2* exP;     // separable propagator, x factor [nx] (includes 1/(nx ny))
     const double2* exS;
     const double2* eyP;     // y factor [ny]

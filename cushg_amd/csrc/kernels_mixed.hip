@@ -161,6 +161,15 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
     // calls of this launch fall in the same heat slice
     VoxRow vox;
     int hv = -1;
+    // per-slice trace: the state before the SSFM step of slice g
+    const bool trace = I.trP && g <= nz - 1 && g % I.trace_every == 0;
+    if (trace && g == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            I.trP[base + q * TPR] = p[q];
+            I.trS[base + q * TPR] = s[q];
+        }
+    }
     if (g > 0) {
         fft_multi<float2, NX, true, 2>(f, my, ROWS * LD, tw, tp);
         if constexpr (THERMAL) {
@@ -173,6 +182,13 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
             for (int q = 0; q < 8; q++) {
                 I.outP[base + q * TPR] = p[q];
                 I.outS[base + q * TPR] = s[q];
+            }
+        }
+        if (trace) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                I.trP[base + q * TPR] = p[q];
+                I.trS[base + q * TPR] = s[q];
             }
         }
     }

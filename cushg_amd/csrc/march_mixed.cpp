@@ -25,6 +25,9 @@ struct MixedHost {
     double dx = 0, dy = 0, dz = 0;
     double Tc = 0;  // temperature at the grid centre of heat slice 0
     float2 *P = nullptr, *S = nullptr, *outP = nullptr, *outS = nullptr;
+    SliceTrace* trace = nullptr;  // per-slice trace (single runs)
+    float2 *trP = nullptr, *trS = nullptr;
+    int pass = 0;                 // thermo-optic pass being marched
     double2 *exP = nullptr, *exS = nullptr, *eyP = nullptr, *eyS = nullptr;
     float2* phase = nullptr;
     double* pol = nullptr;
@@ -183,6 +186,25 @@ static int setup_fields(shg_handle* h, MixedHost& m) {
     return rc;
 }
 
+// Copies the traced state of slice g (written by row pass g) to the host
+// as FP64 and hands it to the trace callback.
+static int deliver_slice(shg_handle* h, MixedHost& m, int g) {
+    std::vector<float2> buf(2 * m.N2);
+    SHG_HIP(hipMemcpyAsync(buf.data(), m.trP, sizeof(float2) * m.N2, hipMemcpyDeviceToHost, h->stream));
+    SHG_HIP(hipMemcpyAsync(buf.data() + m.N2, m.trS, sizeof(float2) * m.N2, hipMemcpyDeviceToHost, h->stream));
+    SHG_HIP(hipStreamSynchronize(h->stream));
+    SliceTrace& t = *m.trace;
+    t.pump.resize(2 * m.N2);
+    t.signal.resize(2 * m.N2);
+    for (size_t i = 0; i < m.N2; i++) {
+        t.pump[2 * i] = buf[i].x;
+        t.pump[2 * i + 1] = buf[i].y;
+        t.signal[2 * i] = buf[m.N2 + i].x;
+        t.signal[2 * i + 1] = buf[m.N2 + i].y;
+    }
+    return t.deliver(g, m.pass);
+}
+
 // The batched z-march: launches A(0) B(0) A(1) ... B(G-1) A(G), G = max nz.
 static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, size_t pool_cap) {
     size_t pool_used = 0;
@@ -246,6 +268,9 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal));
             SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, rows, grid, h->stream, gr.d, g, gr.tw));
         }
+        for (auto* m : act)
+            if (m->trace && m->trace->wants(g, m->nz))
+                if (int rc = deliver_slice(h, *m, g)) return rc;
         if (g == G) break;
         for (auto& kv : colg) {
             Group& gr = kv.second;
@@ -274,14 +299,23 @@ static size_t mixed_bytes(const MixedHost& m) {
                2 * al(sizeof(double2) * m.ny) + al(sizeof(float2) * (size_t)m.nz * 6) + al(sizeof(double) * 4) +
                2 * al(sizeof(MixedInst));
     if (m.thermal) b += heat_bytes(m.p);
+    if (m.trace) b += 2 * al(sizeof(float2) * m.N2);
     return b;
 }
 
 static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& ps,
                           const std::vector<shg_result*>& rs, std::vector<MixedHost>& ms);
 
-int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs) {
+int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std::vector<shg_result*>& rs,
+              SliceTrace* trace) {
     std::vector<MixedHost> ms(ps.size());
+    if (trace) {
+        if (ps.size() != 1) {
+            set_error("a per-slice trace needs a single run");
+            return SHG_EINVAL;
+        }
+        ms[0].trace = trace;
+    }
     const int rc = run_mixed_impl(h, ps, rs, ms);
     for (auto& m : ms) heat_release(m.heat);
     return rc;
@@ -320,6 +354,10 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         m.eyP = h->arena.carve<double2>(m.ny);
         m.eyS = h->arena.carve<double2>(m.ny);
         m.phase = h->arena.carve<float2>((size_t)m.nz * 6);
+        if (m.trace) {
+            m.trP = h->arena.carve<float2>(m.N2);
+            m.trS = h->arena.carve<float2>(m.N2);
+        }
         m.slots = h->arena.carve<double>(4);
         if (!m.slots) {
             set_error("arena exhausted");
@@ -333,6 +371,9 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         d.S = m.S;
         d.outP = m.outP;
         d.outS = m.outS;
+        d.trP = m.trP;
+        d.trS = m.trS;
+        d.trace_every = m.trace ? m.trace->every : 1;
         d.exP = m.exP;
         d.exS = m.exS;
         d.eyP = m.eyP;
@@ -388,6 +429,7 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
             rc = heat_prepare_optics(h, m->p, m->heat, &m->Tc);
             if (rc) return rc;
             m->desc.nn = m->heat.nn;
+            m->pass = go;
             if ((rc = setup_fields(h, *m))) return rc;
             act.push_back(m);
         }

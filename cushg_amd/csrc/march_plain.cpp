@@ -106,7 +106,7 @@ int rk4(shg_handle* h, Plain& m, PlainStage st, const double c[3], double w, con
 
 }  // namespace
 
-int run_plain(shg_handle* h, const shg_params* p, shg_result* r) {
+int run_plain(shg_handle* h, const shg_params* p, shg_result* r, SliceTrace* trace) {
     const auto t0 = std::chrono::steady_clock::now();
     Plain m{};
     m.p = p;
@@ -209,6 +209,16 @@ int run_plain(shg_handle* h, const shg_params* p, shg_result* r) {
             if (s == m.nz - 1) {  // saved before the last slice
                 SHG_HIP(hipMemcpyAsync(m.outP, m.P, sizeof(double2) * m.n2, hipMemcpyDeviceToDevice, h->stream));
                 SHG_HIP(hipMemcpyAsync(m.outS, m.S, sizeof(double2) * m.n2, hipMemcpyDeviceToDevice, h->stream));
+            }
+            if (trace && trace->wants(s, m.nz)) {
+                trace->pump.resize(2 * m.n2);
+                trace->signal.resize(2 * m.n2);
+                SHG_HIP(hipMemcpyAsync(trace->pump.data(), m.P, sizeof(double2) * m.n2, hipMemcpyDeviceToHost,
+                                       h->stream));
+                SHG_HIP(hipMemcpyAsync(trace->signal.data(), m.S, sizeof(double2) * m.n2, hipMemcpyDeviceToHost,
+                                       h->stream));
+                SHG_HIP(hipStreamSynchronize(h->stream));
+                if (int rc = trace->deliver(s, go)) return rc;
             }
             st.step = s;
             st.h = (int)(((long long)s * m.nzh) / m.nz);

@@ -42,6 +42,24 @@ struct Arena {
     }
 };
 
+// Per-slice trace of a single run (shg_run_traced): the callback, the
+// slice stride and host staging for the two fields.
+struct SliceTrace {
+    int every = 1;
+    shg_slice_fn fn = nullptr;
+    void* user = nullptr;
+    std::vector<double> pump, signal;
+    // hands slice `slice` of pass `pass` (already in pump/signal) to fn
+    int deliver(int slice, int pass) {
+        if (fn(slice, pass, pump.data(), signal.data(), user) != 0) {
+            set_error("run cancelled by the slice callback at slice %d of pass %d", slice, pass);
+            return SHG_EINVAL;
+        }
+        return SHG_OK;
+    }
+    bool wants(int slice, int nz) const { return slice <= nz - 1 && slice % every == 0; }
+};
+
 struct KernelTime {
     double ms = 0;
     int launches = 0;

@@ -12,5 +12,4 @@ int shg_set_plan(shg_handle* h, int32_t n, int32_t r, int32_t c) {
     return SHG_OK;
 }
 int shg_autotune(shg_handle*, int32_t, int32_t, int32_t) { return SHG_EUNSUPPORTED; }
-int shg_cli_main(int, const char* const*, const shg_params*, const char*) { return SHG_EUNSUPPORTED; }
 }

@@ -28,12 +28,16 @@ LIB_PATH = os.path.join(_HERE, "lib", "libshg_hip.so")
 EXPORTS = (
     "shg_fft_size_supported", "shg_params_default", "shg_params_set_crystal",
     "shg_refractive_index", "shg_delta_k", "shg_validate", "shg_last_error",
-    "shg_handle_create", "shg_handle_destroy", "shg_run", "shg_run_batch", "shg_adjoint",
+    "shg_handle_create", "shg_handle_destroy", "shg_run", "shg_run_traced", "shg_run_batch", "shg_adjoint",
     "shg_cli_main", "shg_last_kernel_times", "shg_handle_set_profiling", "shg_set_plan",
     "shg_autotune",
 )
 
 _lib = None
+
+# typedef int (*shg_slice_fn)(int32_t slice, int32_t pass, const double* pump,
+#                             const double* signal, void* user)
+SLICE_FN = C.CFUNCTYPE(C.c_int, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
 
 
 class ShgError(RuntimeError):
@@ -67,6 +71,7 @@ def lib():
     L.shg_handle_create.argtypes = [C.c_int32, C.POINTER(H)]
     L.shg_handle_destroy.argtypes = [H]
     L.shg_run.argtypes = [H, P, R]
+    L.shg_run_traced.argtypes = [H, P, C.c_int32, SLICE_FN, C.c_void_p, R]
     L.shg_run_batch.argtypes = [H, P, C.c_int32, R]
     L.shg_adjoint.argtypes = [H, P, C.c_int32, C.POINTER(C.c_double), R]
     L.shg_cli_main.argtypes = [C.c_int, C.POINTER(C.c_char_p), P, C.c_char_p]
@@ -147,6 +152,31 @@ class Handle:
         _check(lib().shg_run(self._h, C.byref(p), C.byref(r)))
         return r, out
 
+    def run_traced(self, p: ShgParams, every: int, fn, fields: bool = False, temperature: bool = False):
+        """shg_run_traced: fn(slice, pass, pump, signal) receives the state
+        before the SSFM step of slices 0, every, ... of every thermo-optic
+        pass as (ny, nx) complex128 arrays (copies); a truthy return value
+        cancels the run.  Returns (ShgResult, dict of output arrays)."""
+        r, out = self._prep_result(p, fields, temperature)
+        n2 = p.nx * p.ny
+        err = []
+
+        def tramp(slice_, pass_, pp, ps_, _user):
+            try:
+                a = np.ctypeslib.as_array(pp, (2 * n2,)).view(np.complex128).reshape(p.ny, p.nx).copy()
+                b = np.ctypeslib.as_array(ps_, (2 * n2,)).view(np.complex128).reshape(p.ny, p.nx).copy()
+                return 1 if fn(slice_, pass_, a, b) else 0
+            except Exception as e:  # an exception in the callback cancels the run
+                err.append(e)
+                return 1
+
+        cb = SLICE_FN(tramp)
+        rc = lib().shg_run_traced(self._h, C.byref(p), every, cb, None, C.byref(r))
+        if err:
+            raise err[0]
+        _check(rc)
+        return r, out
+
     def run_batch(self, ps, fields: bool = False, temperature: bool = False):
         """shg_run_batch over a list of ShgParams; per-instance status in r.status."""
         n = len(ps)
@@ -196,6 +226,16 @@ def save_matrix_complex(A: np.ndarray, name: str, out_dir: str = ".") -> None:
                 f.write("".join(f"{float(v):.20g}\t" for v in row) + "\n")
 
 
+def save_tensor_real(T: np.ndarray, name: str, out_dir: str = ".") -> None:
+    """Text format of saveTensorReal (src/headers/Files.h:169-188):
+    <name>_<z>.dat per z-slice, ny rows of nx tab-terminated values at the
+    default stream precision (6 significant digits)."""
+    for z in range(T.shape[0]):
+        with open(os.path.join(out_dir, f"{name}_{z}.dat"), "w") as f:
+            for row in T[z].astype(np.float32):
+                f.write("".join(f"{float(v):.6g}\t" for v in row) + "\n")
+
+
 class Solver:
     """Mirror of the reference's class Solver (src/headers/Solver.h:116-150)
     over the C ABI; `params` holds what the reference fixes at compile time."""
@@ -209,10 +249,18 @@ class Solver:
         p = self.params.copy()
         p.power_W, p.waist_um, p.focal_um = float(Power), float(waist), float(focalpoint)
         p.T_C, p.T_inf_C, p.T_peltier_C = float(Temp), float(T_inf), float(Tpeltier)
-        r, out = self.handle.run(p, fields=True, temperature=bool(save_temperature))
+        if save_only_last or out_dir is None:
+            r, out = self.handle.run(p, fields=True, temperature=bool(save_temperature))
+        else:
+            # every slice's state before its SSFM step (src/headers/Solver.h:289-294)
+            def dump(s, _pass, pump, signal):
+                save_matrix_complex(pump, f"Pump_{s}", out_dir)
+                save_matrix_complex(signal, f"Signal_{s}", out_dir)
+            r, out = self.handle.run_traced(p, 1, dump, fields=True, temperature=bool(save_temperature))
         if out_dir is not None:
-            if not save_only_last:
-                raise ShgError(abi.SHG_EUNSUPPORTED, "per-slice dumps (save_only_last=0) not provided")
-            save_matrix_complex(out["pump"], "Pump_out", out_dir)
-            save_matrix_complex(out["signal"], "Signal_out", out_dir)
+            if save_only_last:
+                save_matrix_complex(out["pump"], "Pump_out", out_dir)
+                save_matrix_complex(out["signal"], "Signal_out", out_dir)
+            if save_temperature and "T" in out:
+                save_tensor_real(out["T"], "Tfinal", out_dir)
         return r, out

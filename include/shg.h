@@ -189,6 +189,22 @@ SHG_API int shg_handle_destroy(shg_handle* h);
  * buffers are filled in *r. */
 SHG_API int shg_run(shg_handle* h, const shg_params* p, shg_result* r);
 
+/* Per-slice trace callback: the state before the SSFM step of slice
+ * `slice` of thermo-optic pass `pass` (0-based), i.e. what the reference
+ * writes as Pump_<s>/Signal_<s> when save_only_last is false
+ * (src/headers/Solver.h:289-294).  pump/signal: 2*nx*ny interleaved
+ * doubles, valid during the call only.  A non-zero return cancels the run
+ * (shg_run_traced then returns SHG_EINVAL with that message). */
+typedef int (*shg_slice_fn)(int32_t slice, int32_t pass, const double* pump, const double* signal,
+                            void* user);
+/* shg_run with the per-slice trace: fn is called for slices 0, every,
+ * 2*every, ... <= nz-1 of every pass (the reference overwrites its slice
+ * files on each thermo-optic pass, so the last pass's calls are the
+ * reference's files).  every <= 0 means 1.  Each traced slice waits for the
+ * GPU and copies the two fields to the host: a debugging path, not for
+ * timing. */
+SHG_API int shg_run_traced(shg_handle* h, const shg_params* p, int32_t every, shg_slice_fn fn, void* user,
+                           shg_result* r);
 /* n independent runs in one batched execution (replaces the one process
  * per point sweep of src/cuSHG.sh:58-94).  Instances may differ in every
  * parameter, including grid size and nz.  r[i].status holds each

@@ -1,0 +1,141 @@
+"""The C-ABI boundary without a GPU: the library loads, exports every
+function include/shg.h declares, its structs match the header byte for byte
+(checked by compiling the header with gcc) and the Python mirror, its
+defaults are the reference's compile-time constants, and parameter
+validation rejects what the reference's compile-time setup cannot express.
+No call here launches GPU work."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from cushg_amd import abi, shg
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "shg.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    return re.findall(r"SHG_API\s+[\w\s\*]+?\b(shg_\w+)\s*\(", text)
+
+
+def test_header_declares_what_the_binding_expects():
+    assert sorted(header_functions()) == sorted(shg.EXPORTS)
+
+
+def test_library_exports_every_declared_function():
+    L = shg.lib()
+    for name in header_functions():
+        assert hasattr(L, name), name
+    # only the declared API is visible
+    out = subprocess.run(["nm", "-D", "--defined-only", shg.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln and ln.split()[-1].startswith("shg")}
+    assert exported == set(header_functions())
+
+
+def _c_layout(tmp_path, struct, fields):
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("{f} %zu\\n", offsetof({struct}, {f}));\n' for f in fields)
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"shg.h\"\nint main(void){\n"
+                   f'  printf("sizeof %zu\\n", sizeof({struct}));\n{body}  return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    return {ln.split()[0]: int(ln.split()[1]) for ln in out.splitlines()}
+
+
+@pytest.mark.parametrize("struct,cls", [("shg_params", abi.ShgParams), ("shg_result", abi.ShgResult)])
+def test_struct_layout_matches_header(tmp_path, struct, cls):
+    names = [f for f, _ in cls._fields_]
+    lay = _c_layout(tmp_path, struct, names)
+    assert lay["sizeof"] == C.sizeof(cls)
+    for f in names:
+        assert lay[f] == getattr(cls, f).offset, f
+
+
+def test_defaults_are_the_reference_constants():
+    L = shg.lib()
+    p = abi.ShgParams()
+    L.shg_params_default(C.byref(p))
+    q = abi.default_params()
+    for f, _ in abi.ShgParams._fields_:
+        if f == "poling_phase":
+            assert not p.poling_phase
+            continue
+        assert getattr(p, f) == getattr(q, f), f
+    # src/cuSHG.cu:11-13, src/headers/Crystal.h:26-36, src/cuSHG.sh:43-45
+    assert (p.nx, p.ny, p.nz) == (128, 128, 100)
+    assert (p.lx_um, p.ly_um, p.lz_um) == (2000.0, 1000.0, 30000.0)
+    assert p.lambda_p_um == 1.064 and p.Lambda_um == 7.97 and p.deff_um_per_V == 11.0e-6
+    assert p.focal_um == 0.5 * p.lz_um and p.T_inf_C == 25.0 and p.T_peltier_C == p.T_C
+    assert p.max_outer == 20 and p.check_every == 5000 and p.jacobi_tol == 5e-4
+
+
+@pytest.mark.parametrize("crystal", [abi.CRYSTAL_MGO_SPPLT, abi.CRYSTAL_MGO_PPLN])
+def test_crystal_database_matches_mirror_and_oracle(crystal):
+    L = shg.lib()
+    p = abi.default_params()
+    assert L.shg_params_set_crystal(C.byref(p), crystal) == abi.SHG_OK
+    q = abi.default_params()
+    abi.set_crystal(q, crystal)
+    for f in ("deff_um_per_V", "Lambda_um", "k_th", "therm_exp_a", "therm_exp_b", "alpha_p", "alpha_s"):
+        assert getattr(p, f) == getattr(q, f), f
+    for lam in (1.064, 0.532, 1.55):
+        for T in (25.0, 40.0, 56.9):
+            a = L.shg_refractive_index(crystal, lam, T)
+            assert a == pytest.approx(O.refractive_index(crystal, lam, T), rel=1e-15, abs=0)
+            assert a == pytest.approx(abi.refractive_index(crystal, lam, T), rel=1e-14, abs=0)
+    for T in (25.0, 40.0, 55.66):
+        assert L.shg_delta_k(C.byref(p), T) == pytest.approx(O.delta_k(p, T), rel=1e-12, abs=1e-18)
+    assert L.shg_params_set_crystal(C.byref(p), 7) == abi.SHG_EINVAL
+
+
+def _bad(**kw):
+    p = abi.default_params()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(nx=100), "nx=100"),
+    (dict(ny=1024), "ny=1024"),
+    (dict(nz=1), "nz=1"),
+    (dict(thermal=1, nz=100, nz_heat=1), "nz_heat=1"),
+    (dict(lz_um=0.0), "lengths"),
+    (dict(waist_um=-1.0), ""),
+    (dict(power_W=float("nan")), ""),
+    (dict(crystal=5), "crystal"),
+    (dict(scheme=9), "scheme"),
+    (dict(precision=3), "precision"),
+    (dict(thermal=1, heat_solver=4), "heat"),
+    (dict(Lambda_um=0.0), "Lambda"),
+])
+def test_validate_rejects(kw, msg):
+    L = shg.lib()
+    p = _bad(**kw)
+    assert L.shg_validate(C.byref(p)) == abi.SHG_EINVAL
+    assert msg in shg.last_error()
+
+
+def test_validate_accepts_every_supported_size():
+    L = shg.lib()
+    for n in abi.FFT_SIZES:
+        assert L.shg_fft_size_supported(n) == 1
+        p = _bad(nx=n, ny=abi.FFT_SIZES[-1])
+        assert L.shg_validate(C.byref(p)) == abi.SHG_OK
+    for n in (0, 32, 96, 100, 320, 1024):
+        assert L.shg_fft_size_supported(n) == 0
+
+
+def test_null_arguments_are_einval_not_crashes():
+    L = shg.lib()
+    assert L.shg_validate(None) == abi.SHG_EINVAL
+    assert L.shg_run(None, None, None) == abi.SHG_EINVAL
+    assert L.shg_run_batch(None, None, 0, None) == abi.SHG_EINVAL
+    assert L.shg_handle_destroy(None) == abi.SHG_OK
