@@ -2,26 +2,30 @@
 """Benchmark of the SSFM propagation path on one MI355X.
 
 Metric: grid-point * z-steps per second = sum over instances of
-nx*ny*(z-steps executed) / wall time, FP32-mixed batched form, everything
-resident on the device (the pump is initialised on the GPU; only scalar
-results cross PCIe in the timed region).  One benchmark "step" is one
-complete run of the workload: every thermo-optic pass, heat solve and
-z-march of every instance.
+nx*ny*(z-steps executed) / wall time of the fused, batched FP32-field form,
+everything resident on the device (the pump is initialised on the GPU;
+only scalar results cross PCIe in the timed region).  One benchmark "step"
+is one complete run of the workload: every thermo-optic pass, heat solve
+and z-march of every instance.  The timed form is SHG_PREC_FP32_STORAGE
+(FP32 fields in HBM, FP64 arithmetic in registers) by default; the FP32-
+arithmetic form (SHG_PREC_FP32_MIXED) is timed beside it.
 
-Default workload (the configuration the metric's target is quoted on):
-config 5's batch -- 16 THERMAL instances, 512x512 grid, 4000 z-steps,
-50 mm MgO:PPLN, 30 W, heat grid 512x512x400, over seeded (waist, focus,
-oven temperature) triples (cushg_amd.configs.config5_batch).
+Default workload: 16 THERMAL MgO:PPLN instances -- 512x512 grid, 4000
+z-steps, 50 mm crystal, 30 W, heat grid 512x512x400 -- over seeded
+(waist, focus, oven temperature) triples
+(cushg_amd.configs.ppln512_thermal_batch).
 
 The JSON line also carries:
-  roofline      dominant kernel: algorithmic bytes per launch / average
-                launch time from HIP events on the library's stream;
-                `traffic` = HBM bytes per launch from the committed
-                rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/).
-  fp64_plain    the same instances in the plain FP64 GPU form (which runs
-                one instance at a time), timed on the first instance.
-  cpu_baseline  the FP64 oracle (OpenMP, all host threads) on a bounded
-                sample of the first instance.
+  roofline        dominant kernel: algorithmic bytes per launch / average
+                  launch time from HIP events on the library's stream;
+                  `traffic` = HBM bytes per launch from the committed
+                  rocprofv3 counter summary (profiles/pmc_traffic.json).
+  fp64_plain      the plain FP64 GPU form (one instance at a time) over
+                  every instance: its throughput, and
+  eta_rel_err_vs_fp64  the eta error of the timed form against it.
+  other_fused_form     the other FP32-field form's value and eta error.
+  cpu_baseline    the FP64 oracle (OpenMP, all host threads) on a bounded
+                  sample of the first instance.
 """
 from __future__ import annotations
 
@@ -60,21 +64,21 @@ def rowpass_thermal_march_bytes(p) -> float:
 
 def workload(name: str):
     from cushg_amd import configs
-    if name == "cfg5b16":
-        return configs.config5_batch(16), ("16 x THERMAL 512x512, 4000 z-steps, 50 mm MgO:PPLN, 30 W, "
+    if name == "ppln512t-b16":
+        return configs.ppln512_thermal_batch(16), ("16 x THERMAL 512x512, 4000 z-steps, 50 mm MgO:PPLN, 30 W, "
                                            "heat 512x512x400, seeded waist/focus/oven")
-    if name == "cfg5":
-        return [configs.config5()], "THERMAL 512x512, 4000 z-steps, 50 mm MgO:PPLN, 30 W, 50 um, heat 512x512x400"
-    if name == "cfg2":
-        return [configs.config2()], "THERMAL 256x256, 1000 z-steps, 50 mm sPPLT, 10 W, 40 um, heat 256x256x100, oven 40 C"
-    if name == "cfg2-optics":
-        p = configs.config2()
+    if name == "ppln512t":
+        return [configs.ppln512_thermal()], "THERMAL 512x512, 4000 z-steps, 50 mm MgO:PPLN, 30 W, 50 um, heat 512x512x400"
+    if name == "spplt256t":
+        return [configs.spplt256_thermal()], "THERMAL 256x256, 1000 z-steps, 50 mm sPPLT, 10 W, 40 um, heat 256x256x100, oven 40 C"
+    if name == "spplt256":
+        p = configs.spplt256_thermal()
         p.thermal = 0
         return [p], "256x256, 1000 z-steps, 50 mm sPPLT, 10 W, 40 um, THERMAL off"
-    if name == "cfg3":
-        return configs.config3(seed=0, n=64), "64-instance sweep, grids 128..512, 500-2000 z-steps, THERMAL off"
-    if name == "cfg1":
-        return [configs.config1()], "64x64, 100 z-steps, 30 mm sPPLT, 1 W, 30 um"
+    if name == "sweep64":
+        return configs.sweep_batch(seed=0, n=64), "64-instance sweep, grids 128..512, 500-2000 z-steps, THERMAL off"
+    if name == "spplt64":
+        return [configs.spplt64()], "64x64, 100 z-steps, 30 mm sPPLT, 1 W, 30 um"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -113,14 +117,18 @@ def kernel_roofline(h, ps):
     if not kt:
         return None, kt
     name, (ms, n) = max(kt.items(), key=lambda kv: kv[1][0])
+    from cushg_amd import abi
     parts = name.split("_")
     base, size = parts[0], int(parts[1])
-    thermal = name.endswith("_thermal")
+    thermal = "_thermal" in name
+    f64 = name.endswith("_f64")
     if base not in BYTES_PER_PT:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}, kt
     total = 0.0
     for p, r in zip(ps, res):
+        if (p.precision == abi.PREC_FP32_STORAGE) != f64:
+            continue
         if base == "rowpass" and p.nx == size and bool(p.thermal) == thermal:
             if thermal:
                 total += r.outer_iters * rowpass_thermal_march_bytes(p)
@@ -170,13 +178,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg5b16")
+    ap.add_argument("--workload", default="ppln512t-b16")
     ap.add_argument("--nz", type=int, default=0, help="override z-steps, scaling the heat grid's z-slices with it (profiling runs)")
     ap.add_argument("--max-outer", type=int, default=0, help="override thermo-optic passes (profiling runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--precision", choices=("storage", "mixed"), default="storage",
+                    help="fused form timed as the value: FP32 fields with FP64 arithmetic (storage) or "
+                         "FP32 arithmetic (mixed); the other is reported beside it")
+    ap.add_argument("--no-alt", action="store_true", help="skip the other fused form")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -203,7 +215,10 @@ def main():
     from cushg_amd import abi, shg
 
     ps, desc = workload(args.workload)
+    prec = abi.PREC_FP32_STORAGE if args.precision == "storage" else abi.PREC_FP32_MIXED
+    alt_prec = abi.PREC_FP32_MIXED if prec == abi.PREC_FP32_STORAGE else abi.PREC_FP32_STORAGE
     for p in ps:
+        p.precision = prec
         if args.nz:
             if p.nz_heat > 0:  # keep the optical slices per heat slab
                 p.nz_heat = max(1, round(p.nz_heat * args.nz / p.nz))
@@ -226,20 +241,45 @@ def main():
     roof, kt = (None, {}) if args.no_roofline else kernel_roofline(h, ps)
 
     extra = {}
-    if not args.no_fp64:
-        q = ps[0].copy()
-        q.precision = abi.PREC_FP64_PLAIN
-        run_once(h, [q])
+    alt = None
+    if not args.no_alt:
+        qs = [p.copy() for p in ps]
+        for q in qs:
+            q.precision = alt_prec
+        run_once(h, qs)
         t = time.perf_counter()
-        r64 = run_once(h, [q])
-        d64 = time.perf_counter() - t
-        v64 = pts_steps([q], r64) / d64
+        ralt = run_once(h, qs)
+        dalt = time.perf_counter() - t
+        alt = {"precision": "mixed" if alt_prec == abi.PREC_FP32_MIXED else "storage",
+               "value": round(pts_steps(qs, ralt) / dalt, 1), "unit": UNIT, "ms_per_step": round(dalt * 1e3, 3)}
+        print(f"bench: other fused form {dalt:.2f} s", file=sys.stderr, flush=True)
+    if not args.no_fp64:
+        # the plain FP64 form on every instance (it runs them one at a time):
+        # its throughput, and the eta error of the fused forms against it
+        qs = [p.copy() for p in ps]
+        for q in qs:
+            q.precision = abi.PREC_FP64_PLAIN
+        run_once(h, qs[:1])
+        d64, r64 = 0.0, []
+        for q in qs:
+            t = time.perf_counter()
+            r64 += run_once(h, [q])
+            d64 += time.perf_counter() - t
+        v64 = pts_steps(qs, r64) / d64
+        errs = [abs(a.eta - b.eta) / abs(b.eta) for a, b in zip(res, r64)]
         extra["fp64_plain"] = {"value": round(v64, 1), "unit": UNIT,
-                               "sample": "first instance (the plain form runs instances one at a time)",
-                               "ms_per_instance": round(d64 * 1e3, 1), "eta": r64[0].eta,
-                               "eta_rel_err_fp32_vs_fp64": abs(res[0].eta - r64[0].eta) / abs(r64[0].eta)}
+                               "sample": f"all {len(qs)} instances, one at a time", "s_total": round(d64, 2),
+                               "eta": [round(r.eta, 10) for r in r64[:4]]}
+        extra["eta_rel_err_vs_fp64"] = {"max": max(errs), "mean": sum(errs) / len(errs), "bar": 1e-5,
+                                        "instances": len(errs)}
         extra["speedup_vs_fp64_plain"] = round(value / v64, 2)
-        print(f"bench: fp64 plain {d64:.2f} s", file=sys.stderr, flush=True)
+        if alt is not None:
+            ae = [abs(a.eta - b.eta) / abs(b.eta) for a, b in zip(ralt, r64)]
+            alt["eta_rel_err_vs_fp64_max"] = max(ae)
+            alt["speedup_vs_fp64_plain"] = round(alt["value"] / v64, 2)
+        print(f"bench: fp64 plain {d64:.2f} s, eta rel err max {max(errs):.2e}", file=sys.stderr, flush=True)
+    if alt is not None:
+        extra["other_fused_form"] = alt
 
     line = {
         "metric": METRIC,
@@ -252,10 +292,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
-        "dtype_detail": "fields, FFTs, RK4 in fp32; dk*z phase, reductions and heat residual in fp64",
+        "dtype": "f64" if prec == abi.PREC_FP32_STORAGE else "fp32",
+        "dtype_detail": ("fields fp32 in HBM (8 B per complex point); FFTs, propagator, phase and RK4 in "
+                         "f64 registers; reductions and heat residual f64, heat corrections fp32 GEMMs"
+                         if prec == abi.PREC_FP32_STORAGE else
+                         "fields, FFT butterflies, RK4 in fp32; twiddle/propagator products, dk*z phase, "
+                         "reductions and heat residual in f64"),
         "data": "synthetic (Gaussian pump from parameters)",
         "config": {"workload": f"{args.workload}: {desc}", "instances": len(ps),
+                   "form": "FP32 batched, " + ("FP32 fields / FP64 arithmetic (SHG_PREC_FP32_STORAGE)"
+                                               if prec == abi.PREC_FP32_STORAGE else
+                                               "FP32 arithmetic (SHG_PREC_FP32_MIXED)"),
                    "z_steps_per_run": int(sum(r.steps_done for r in res)),
                    "eta": [round(r.eta, 9) for r in res[:4]],
                    "peak_dT_K": [round(r.peak_dT_K, 6) for r in res[:4]],

@@ -34,6 +34,7 @@ SCHEME_REFERENCE = 0
 SCHEME_STRANG = 1
 PREC_FP32_MIXED = 0
 PREC_FP64_PLAIN = 1
+PREC_FP32_STORAGE = 2
 HEAT_JACOBI_REFERENCE = 0
 HEAT_DIRECT = 1
 

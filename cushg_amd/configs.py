@@ -22,7 +22,7 @@ def _base(n: int, nz: int, lz_um: float) -> ShgParams:
     return p
 
 
-def config1() -> ShgParams:
+def spplt64() -> ShgParams:
     """Plain SSFM, CPU-runnable: 64x64 over 0.8 mm, 100 slices, 30 mm
     MgO:sPPLT, Lambda = 7.97 um, 1 W, 30 um waist at the centre, T = 55.66 C
     (src/cuSHG.sh:45), no absorption, THERMAL off."""
@@ -35,7 +35,7 @@ def config1() -> ShgParams:
     return p
 
 
-def config2() -> ShgParams:
+def spplt256_thermal() -> ShgParams:
     """THERMAL: 256x256, 1000 slices, 50 mm, 10 W, 40 um waist,
     alpha_p = 0.0002/cm, alpha_SH = 0.02/cm (linear absorption only),
     heat grid 256x256x100, oven 40 C, heat tolerance 1e-8 K.  The grating is
@@ -67,7 +67,7 @@ def splitmix64(seed: int):
         yield (z >> 11) / float(1 << 53)
 
 
-def config3(seed: int = 0, n: int = 64, scale_steps: float = 1.0):
+def sweep_batch(seed: int = 0, n: int = 64, scale_steps: float = 1.0):
     """Batched sweep: n instances, seeded uniform P 0.1-20 W, waist 15-80 um,
     focus -0.5L..+0.5L about the centre, oven 30-60 C, grid from
     {128,192,256,384,512}, 500-2000 slices (times scale_steps)."""
@@ -87,7 +87,7 @@ def config3(seed: int = 0, n: int = 64, scale_steps: float = 1.0):
     return out
 
 
-def config4(nz: int = 2000, n: int = 256, seed: int = 0):
+def adjoint256(nz: int = 2000, n: int = 256, seed: int = 0):
     """Adjoint: 256x256, 2000 slices, 30 mm MgO:sPPLT, 10 W, 40 um, THERMAL off,
     poling phases ~ N(0, 0.05 rad) from SplitMix64 + Box-Muller (seed 0)."""
     p = _base(n, nz, 30000.0)
@@ -106,7 +106,7 @@ def config4(nz: int = 2000, n: int = 256, seed: int = 0):
     return p
 
 
-def config5() -> ShgParams:
+def ppln512_thermal() -> ShgParams:
     """Full coupled case: 512x512, 4000 slices, 50 mm MgO:PPLN (literature
     constants), 30 W, 50 um, THERMAL with a 512x512x400 heat grid."""
     p = _base(512, 4000, 50000.0)
@@ -121,12 +121,12 @@ def config5() -> ShgParams:
     return p
 
 
-def config5_batch(n: int = 16, seed: int = 5):
-    """Config 5's batch: n THERMAL 512x512x4000 instances (heat 512x512x400)
+def ppln512_thermal_batch(n: int = 16, seed: int = 5):
+    """n THERMAL MgO:PPLN 512x512x4000 instances (heat 512x512x400)
     over seeded (waist, focus, oven temperature) triples around the nominal
     point: waist U(40, 60) um, focus U(0.4 L, 0.6 L), oven U(38, 42) C, the
     grating fixed at the 40 C first-order period."""
-    base = config5()
+    base = ppln512_thermal()
     r = splitmix64(seed)
     out = []
     for _ in range(n):

@@ -134,7 +134,8 @@ int shg_validate(const shg_params* p) {
         set_error("unknown scheme %d", p->scheme);
         return SHG_EINVAL;
     }
-    if (p->precision != SHG_PREC_FP32_MIXED && p->precision != SHG_PREC_FP64_PLAIN) {
+    if (p->precision != SHG_PREC_FP32_MIXED && p->precision != SHG_PREC_FP64_PLAIN &&
+        p->precision != SHG_PREC_FP32_STORAGE) {
         set_error("unknown precision %d", p->precision);
         return SHG_EINVAL;
     }
@@ -313,7 +314,5 @@ int shg_handle_set_profiling(shg_handle* h, int32_t on) {
     h->profiling = on != 0;
     return SHG_OK;
 }
-
-int shg_set_plan(shg_handle* h, int32_t n, int32_t rows_per_wg, int32_t cols_per_wg);
 
 }  // extern "C"

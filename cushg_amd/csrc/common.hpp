@@ -98,4 +98,35 @@ __device__ __forceinline__ float2 expi_fast(double ph) {
     return make_float2((q == 1 || q == 2) ? -re : re, (q >= 2) ? -im : im);
 }
 
+// FP64 counterpart of expi_fast: the same quarter-turn reduction with the
+// Taylor series of sin and cos through a^15 and a^16 in FP64 (truncation
+// error below 5e-17 for |a| <= pi/4).
+__device__ __forceinline__ double2 expi_fast64(double ph) {
+    const double t = ph * (0.5 / kPi);
+    const double n = rint(4.0 * t);
+    const double a = (t - 0.25 * n) * (2.0 * kPi);
+    const double x2 = a * a;
+    double s = 1.0 / 1307674368000.0;  // 1/15!
+    s = s * -x2 + 1.0 / 6227020800.0;  // 1/13!
+    s = s * -x2 + 1.0 / 39916800.0;
+    s = s * -x2 + 1.0 / 362880.0;
+    s = s * -x2 + 1.0 / 5040.0;
+    s = s * -x2 + 1.0 / 120.0;
+    s = s * -x2 + 1.0 / 6.0;
+    s = a * (1.0 - x2 * s);
+    double c = 1.0 / 20922789888000.0;  // 1/16!
+    c = c * -x2 + 1.0 / 87178291200.0;  // 1/14!
+    c = c * -x2 + 1.0 / 479001600.0;
+    c = c * -x2 + 1.0 / 3628800.0;
+    c = c * -x2 + 1.0 / 40320.0;
+    c = c * -x2 + 1.0 / 720.0;
+    c = c * -x2 + 1.0 / 24.0;
+    c = c * -x2 + 0.5;
+    c = 1.0 - x2 * c;
+    const int q = (int)(n - 4.0 * floor(0.25 * n));
+    const double re = (q & 1) ? s : c;
+    const double im = (q & 1) ? c : s;
+    return make_double2((q == 1 || q == 2) ? -re : re, (q >= 2) ? -im : im);
+}
+
 }  // namespace shg

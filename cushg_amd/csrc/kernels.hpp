@@ -5,8 +5,10 @@
 
 namespace shg {
 
-// One instance of the FP32-mixed march, as the kernels see it.  Lives in
-// device memory (an array per launch group, indexed by blockIdx.z).
+// One instance of a fused march (both FP32-field forms), as the kernels
+// see it.  Lives in device memory (an array per launch group, indexed by
+// blockIdx.z).  Constants are FP64; the FP32-arithmetic kernels round them
+// once at use.
 struct MixedInst {
     float2* P;              // pump field [ny][nx]
     float2* S;              // SH field   [ny][nx]
@@ -14,37 +16,36 @@ struct MixedInst {
     float2* outS;
     float2* trP;            // per-slice trace: state before slice g's SSFM step (or null)
     float2* trS;
-    int trace_every;
     const double2* exP;     // separable propagator, x factor [nx] (includes 1/(nx ny))
     const double2* exS;
     const double2* eyP;     // y factor [ny]
     const double2* eyS;
-    const float2* phase;    // non-THERMAL: exp(i phi) [nz][2 calls][3 offsets]
+    const double2* phase;   // non-THERMAL: exp(i phi) [nz][2 calls][3 offsets]
     const float2* nn;       // THERMAL: (n_p(T), n_s(T)) per heat voxel [nzh][ny][nx]
     const double* dkbase;   // THERMAL: running dk integral = dkbase[h] + s*dk[h]
     const double* dk;       // THERMAL: dk(T) [nzh][ny][nx]
     float* Q;               // THERMAL: heat source [nzh][ny][nx]
     const double* poling;   // THERMAL: per-slice poling phase [nz] or null
     double dz;
+    double kp, ks, bfac;    // non-THERMAL couplings at T_C
+    double kp0, ks0, bf0;   // THERMAL: kp = kp0/n_p(T), ks = ks0/n_s(T), bfac = bf0 n_s(T)
+    double ap_h, as_h;      // alpha/2
+    double c1, c2, c3, w;   // RK4 stage coefficients and final weight
+    double off[2][3];       // z offsets (units of dz) per call
+    double hce;             // c eps0 / 2
+    double alpha_p, alpha_s, beta_s;
     int nx, ny, nz, nzh;
     int crystal;
-    float lp, ls;
-    float kp, ks, bfac;     // non-THERMAL couplings at T_C
-    float kp0, ks0, bf0;    // THERMAL: kp = kp0/n_p(T), ks = ks0/n_s(T), bfac = bf0 n_s(T)
-    float ap_h, as_h;       // alpha/2
-    float c1, c2, c3, w;    // RK4 stage coefficients and final weight
-    float off[2][3];        // z offsets (units of dz) per call
-    float hce;              // c eps0 / 2
-    float alpha_p, alpha_s, beta_s;
+    int trace_every;
 };
 
-// FP32-mixed pipeline (kernels_mixed.hip)
-hipError_t launch_rowpass(int nx, bool thermal, int rows_pw, dim3 grid, hipStream_t st,
+// fused passes (kernels_mixed.hip); fp64: double2 arithmetic in registers
+hipError_t launch_rowpass(int nx, bool thermal, bool fp64, int rows_pw, dim3 grid, hipStream_t st,
                           const MixedInst* insts, int g, const double2* tw);
-hipError_t launch_colpass(int ny, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
+hipError_t launch_colpass(int ny, bool fp64, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
                           int g, const double2* tw);
-int rowpass_rows_default(int nx);
-int colpass_cols_default(int ny);
+int rowpass_rows_default(int nx, bool fp64);
+int colpass_cols_default(int ny, bool fp64);
 bool colpass_cols_supported(int ny, int cols);
 bool rowpass_rows_supported(int nx, int rows);
 

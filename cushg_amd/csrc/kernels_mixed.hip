@@ -1,4 +1,4 @@
-// kernels_mixed.hip -- FP32-mixed fused SSFM passes for gfx950.
+// kernels_mixed.hip -- fused SSFM passes of the FP32-field forms (gfx950).
 //
 // One SSFM slice of the reference (src/headers/Solver.h:267-273) is
 //   A <- R(A, z_s);  A <- IFFT2(Prop . FFT2(A)) / (nx ny);  A <- R(A, z_s)
@@ -12,38 +12,102 @@
 // grid point.  The 1/(nx ny) scale of cuFFTscale (src/headers/Efields.h:15-29)
 // is folded into the propagator table.
 //
-// Precision: fields, FFTs and RK4 arithmetic in FP32; the QPM phase dk*z is
-// formed in FP64 and reduced to a quarter turn before the FP32 sin/cos
-// series (expi_fast).
+// Fields are FP32 in device memory in both forms.  The in-register
+// arithmetic type C is float2 (SHG_PREC_FP32_MIXED: FFT butterflies, RK4
+// and the phase factor in FP32; twiddle and propagator products in FP64,
+// rounded once) or double2 (SHG_PREC_FP32_STORAGE: everything between the
+// FP32 load and the FP32 store in FP64).  The QPM phase dk*z is FP64 in
+// both and reduced to a quarter turn before the sin/cos series.
 #include "fft.hpp"
 #include "kernels.hpp"
 
 namespace shg {
 
-// dA/dz of src/headers/Solver.h:57-80 (and :33-55 for THERMAL), FP32:
+template <typename C> using re_t = decltype(C{}.x);
+
+template <typename C> __device__ __forceinline__ C from_store(float2 v) { return mk<C>(v.x, v.y); }
+
+// 32-bit integer hash (Wellons' "lowbias32" finaliser): a counter-based
+// random stream keyed by element index, step and store site.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// FP64 -> FP32 with stochastic rounding: 29 random bits are added below the
+// 23-bit FP32 mantissa of the FP64 pattern and the value is truncated, so
+// it rounds up in magnitude with probability equal to the dropped fraction
+// of an FP32 ulp.  Round-to-nearest would be biased for a component whose
+// change per slice is below half an ulp (it would be stored unchanged slice
+// after slice); stochastic rounding is unbiased for every value.
+__device__ __forceinline__ float sr_f32(double v, uint32_t rnd) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    b += (unsigned long long)(rnd >> 3);
+    b &= ~((1ull << 29) - 1);
+    return (float)__longlong_as_double((long long)b);
+}
+
+// Store keys: site (0 row-pass state, 1 column-pass state, 2 saved output,
+// 3 trace), field (0 pump, 1 SH) and step g.
+__device__ __forceinline__ uint32_t store_key(int site, int field, int g) {
+    return mix32(0x9e3779b9U * (uint32_t)(g + 1) ^ (uint32_t)(site << 28 | field << 27));
+}
+__device__ __forceinline__ float2 to_store(float2 v, size_t, uint32_t) { return v; }
+__device__ __forceinline__ float2 to_store(double2 v, size_t idx, uint32_t key) {
+    const uint32_t k = key ^ (uint32_t)(idx * 2);
+    return make_float2(sr_f32(v.x, mix32(k)), sr_f32(v.y, mix32(k + 1)));
+}
+
+template <typename C> __device__ __forceinline__ C expi(double ph);
+template <> __device__ __forceinline__ float2 expi<float2>(double ph) { return expi_fast(ph); }
+template <> __device__ __forceinline__ double2 expi<double2>(double ph) { return expi_fast64(ph); }
+
+// 1/n for an FP32 index n: hardware reciprocal, refined by Newton steps to
+// the precision of R
+template <typename R> __device__ __forceinline__ R recip(float n);
+template <> __device__ __forceinline__ float recip<float>(float n) {
+    const float r = __builtin_amdgcn_rcpf(n);
+    return r * __builtin_fmaf(-n, r, 2.0f);
+}
+template <> __device__ __forceinline__ double recip<double>(float n) {
+    const double nd = n;
+    double r = __builtin_amdgcn_rcpf(n);
+    r = r * (2.0 - nd * r);
+    return r * (2.0 - nd * r);
+}
+
+// dA/dz of src/headers/Solver.h:57-80 (and :33-55 for THERMAL):
 //   dP = i kp S conj(P) conj(e) - alpha_p/2 P
 //   dS = i ks P^2 e - (alpha_s/2 + bfac/2 |S|^2) S
-__device__ __forceinline__ void dadz(float2 P, float2 S, float2 e, float kp, float ks, float aph,
-                                     float ash, float bfac, float2& dP, float2& dS) {
-    const float2 spc = cmul(cmul(S, conjc(P)), conjc(e));
-    dP = make_float2(-kp * spc.y - aph * P.x, kp * spc.x - aph * P.y);
-    const float2 ppe = cmul(cmul(P, P), e);
-    const float g = ash + 0.5f * bfac * abs2(S);
-    dS = make_float2(-ks * ppe.y - g * S.x, ks * ppe.x - g * S.y);
+template <typename C, typename R = re_t<C>>
+__device__ __forceinline__ void dadz(C P, C S, C e, R kp, R ks, R aph, R ash, R bfac, C& dP, C& dS) {
+    const C spc = cmul(cmul(S, conjc(P)), conjc(e));
+    dP = mk<C>(-kp * spc.y - aph * P.x, kp * spc.x - aph * P.y);
+    const C ppe = cmul(cmul(P, P), e);
+    const R g = ash + R(0.5) * bfac * abs2(S);
+    dS = mk<C>(-ks * ppe.y - g * S.x, ks * ppe.x - g * S.y);
 }
+
+// Scheme constants of one R call in the arithmetic type.
+template <typename R> struct RkConst {
+    R c1, c2, c3, w, aph, ash;
+};
 
 // RK4 call of src/headers/Solver.h:231-257 on one point; stage inputs
 // A + c*k with c = (c1, c2, c3), z offsets e[0], e[1] (stages 2 and 3), e[2].
-__device__ __forceinline__ void rk4_point(float2& P, float2& S, const float2 (&e)[3], float kp,
-                                          float ks, float aph, float ash, float bfac, float c1,
-                                          float c2, float c3, float w) {
-    float2 k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
-    dadz(P, S, e[0], kp, ks, aph, ash, bfac, k1p, k1s);
-    dadz(P + c1 * k1p, S + c1 * k1s, e[1], kp, ks, aph, ash, bfac, k2p, k2s);
-    dadz(P + c2 * k2p, S + c2 * k2s, e[1], kp, ks, aph, ash, bfac, k3p, k3s);
-    dadz(P + c3 * k3p, S + c3 * k3s, e[2], kp, ks, aph, ash, bfac, k4p, k4s);
-    P = P + w * (k1p + 2.0f * k2p + 2.0f * k3p + k4p);
-    S = S + w * (k1s + 2.0f * k2s + 2.0f * k3s + k4s);
+template <typename C, typename R = re_t<C>>
+__device__ __forceinline__ void rk4_point(C& P, C& S, const C (&e)[3], R kp, R ks, R bfac, const RkConst<R>& k) {
+    C k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
+    dadz(P, S, e[0], kp, ks, k.aph, k.ash, bfac, k1p, k1s);
+    dadz(P + k.c1 * k1p, S + k.c1 * k1s, e[1], kp, ks, k.aph, k.ash, bfac, k2p, k2s);
+    dadz(P + k.c2 * k2p, S + k.c2 * k2s, e[1], kp, ks, k.aph, k.ash, bfac, k3p, k3s);
+    dadz(P + k.c3 * k3p, S + k.c3 * k3s, e[2], kp, ks, k.aph, k.ash, bfac, k4p, k4s);
+    P = P + k.w * (k1p + R(2) * k2p + R(2) * k3p + k4p);
+    S = S + k.w * (k1s + R(2) * k2s + R(2) * k3s + k4s);
 }
 
 __device__ __forceinline__ int heat_slice(int s, int nz, int nzh) {
@@ -80,51 +144,49 @@ __device__ __forceinline__ void load_vox(VoxRow& v, const MixedInst& I, int h, i
 // `step` on the 8 points x = tp + q*TPR of row y.  With THERMAL the values
 // of the slice's heat voxels come in `vox` (heat slice h of `step`) and the
 // heat source of the slice is deposited after the second call.
-template <int NX, bool THERMAL>
-__device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const MixedInst& I,
-                                       int step, int call, int y, int tp, const VoxRow& vox, int h,
-                                       bool deposit_q) {
+template <int NX, bool THERMAL, typename C>
+__device__ __forceinline__ void nl_row(C (&p)[8], C (&s)[8], const MixedInst& I, int step, int call, int y,
+                                       int tp, const VoxRow& vox, int h, bool deposit_q) {
+    using R = re_t<C>;
     constexpr int TPR = NX / 8;
-    const float c1 = I.c1, c2 = I.c2, c3 = I.c3, w = I.w;
-    const float aph = I.ap_h, ash = I.as_h;
+    const RkConst<R> k{(R)I.c1, (R)I.c2, (R)I.c3, (R)I.w, (R)I.ap_h, (R)I.as_h};
     if constexpr (!THERMAL) {
-        const float2* ph = I.phase + (size_t)(step * 2 + call) * 3;
-        const float2 e[3] = {ph[0], ph[1], ph[2]};
-        const float kp = I.kp, ks = I.ks, bfac = I.bfac;
+        const double2* ph = I.phase + (size_t)(step * 2 + call) * 3;
+        const C e[3] = {mk<C>((R)ph[0].x, (R)ph[0].y), mk<C>((R)ph[1].x, (R)ph[1].y),
+                        mk<C>((R)ph[2].x, (R)ph[2].y)};
+        const R kp = (R)I.kp, ks = (R)I.ks, bfac = (R)I.bfac;
 #pragma unroll
-        for (int q = 0; q < 8; q++) rk4_point(p[q], s[q], e, kp, ks, aph, ash, bfac, c1, c2, c3, w);
+        for (int q = 0; q < 8; q++) rk4_point(p[q], s[q], e, kp, ks, bfac, k);
     } else {
         const double pol = I.poling ? I.poling[step] : 0.0;
         const double zsc = I.dz / ((double)step + 1.0);
         const double o0 = (double)step + I.off[call][0];
         float* Qr = I.Q + (size_t)h * I.nx * I.ny + (size_t)y * I.nx;
         const bool first = step == 0 || heat_slice(step - 1, I.nz, I.nzh) != h;
-        const float qw = 1.0f / (float)heat_count(h, I.nz, I.nzh);
+        const R qw = R(1) / (R)heat_count(h, I.nz, I.nzh);
+        const R kp0 = (R)I.kp0, ks0 = (R)I.ks0, bf0 = (R)I.bf0, hce = (R)I.hce;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int x = tp + q * TPR;
-            const float np = vox.n[q].x, ns = vox.n[q].y;  // n_p(T), n_s(T) of this voxel
-            // 1/n by the hardware reciprocal refined with one Newton step
-            float rp = __builtin_amdgcn_rcpf(np), rs = __builtin_amdgcn_rcpf(ns);
-            rp = rp * __builtin_fmaf(-np, rp, 2.0f);
-            rs = rs * __builtin_fmaf(-ns, rs, 2.0f);
-            const float kp = I.kp0 * rp, ks = I.ks0 * rs, bfac = I.bf0 * ns;
+            const float npf = vox.n[q].x, nsf = vox.n[q].y;  // n_p(T), n_s(T) of this voxel
+            const R np = npf, ns = nsf;
+            const R kp = kp0 * recip<R>(npf), ks = ks0 * recip<R>(nsf), bfac = bf0 * ns;
             // running dk integral (src/headers/PhaseMatching.h:27-44) and the
             // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50;
             // the three offsets are 0.25 dz apart in both schemes, so
             // e(off + 1/4) = e(off) exp(i DKint dz / (4 (s + 1))).
             const double dki = vox.dkb[q] + (double)step * vox.dk[q];
-            const float2 e0 = expi_fast(dki * o0 * zsc + pol);
-            const float2 ed = expi_fast(0.25 * dki * zsc);
-            const float2 e1 = cmul(e0, ed);
-            const float2 e[3] = {e0, e1, cmul(e1, ed)};
-            rk4_point(p[q], s[q], e, kp, ks, aph, ash, bfac, c1, c2, c3, w);
+            const C e0 = expi<C>(dki * o0 * zsc + pol);
+            const C ed = expi<C>(0.25 * dki * zsc);
+            const C e1 = cmul(e0, ed);
+            const C e[3] = {e0, e1, cmul(e1, ed)};
+            rk4_point(p[q], s[q], e, kp, ks, bfac, k);
             if (deposit_q) {
                 // Q = a_p I_p + a_s I_s + b_s I_s^2 -- src/headers/Tfield.h:260-276
-                const float Ip = I.hce * np * abs2(p[q]);
-                const float Is = I.hce * ns * abs2(s[q]);
-                const float qv = (I.alpha_p * Ip + I.alpha_s * Is + I.beta_s * Is * Is) * qw;
-                Qr[x] = first ? qv : Qr[x] + qv;
+                const R Ip = hce * np * abs2(p[q]);
+                const R Is = hce * ns * abs2(s[q]);
+                const R qv = ((R)I.alpha_p * Ip + (R)I.alpha_s * Is + (R)I.beta_s * Is * Is) * qw;
+                Qr[x] = first ? (float)qv : (float)((R)Qr[x] + qv);
             }
         }
     }
@@ -135,27 +197,27 @@ __device__ __forceinline__ void nl_row(float2 (&p)[8], float2 (&s)[8], const Mix
 //   0 < g < nz   : row IFFT -> R_end(g-1) [-> save state if g-1 == nz-2] -> R_start(g) -> row FFT
 //   g == nz      : row IFFT -> R_end(nz-1) -> spatial fields stored
 // grid (ceil(max_ny/ROWS), 1, n_instances), block (NX/8, ROWS)
-template <int NX, bool THERMAL, int ROWS>
+template <int NX, bool THERMAL, int ROWS, typename C>
 __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __restrict__ insts, int g,
                                                            const double2* __restrict__ tw) {
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
-    __shared__ float2 lds[2 * ROWS * LD];
+    __shared__ C lds[2 * ROWS * LD];
     const MixedInst& I = insts[blockIdx.z];
     const int nz = I.nz;
     if (g > nz) return;
     const int y = blockIdx.x * ROWS + threadIdx.y;
     if (blockIdx.x * ROWS >= I.ny) return;
     const int tp = threadIdx.x;
-    float2* my = lds + threadIdx.y * LD;  // pump buffer; SH buffer ROWS*LD further
+    C* my = lds + threadIdx.y * LD;  // pump buffer; SH buffer ROWS*LD further
     const size_t base = (size_t)y * NX + tp;
-    float2 f[2][8];  // f[0] pump, f[1] SH: transformed together (shared barriers)
-    float2(&p)[8] = f[0];
-    float2(&s)[8] = f[1];
+    C f[2][8];  // f[0] pump, f[1] SH: transformed together (shared barriers)
+    C(&p)[8] = f[0];
+    C(&s)[8] = f[1];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        p[q] = I.P[base + q * TPR];
-        s[q] = I.S[base + q * TPR];
+        p[q] = from_store<C>(I.P[base + q * TPR]);
+        s[q] = from_store<C>(I.S[base + q * TPR]);
     }
     // heat-voxel values of the slab in registers, loaded once when both R
     // calls of this launch fall in the same heat slice
@@ -166,29 +228,33 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
     if (trace && g == 0) {
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            I.trP[base + q * TPR] = p[q];
-            I.trS[base + q * TPR] = s[q];
+            const size_t i = base + q * TPR;
+            I.trP[i] = to_store(p[q], i, store_key(3, 0, g));
+            I.trS[i] = to_store(s[q], i, store_key(3, 1, g));
         }
     }
     if (g > 0) {
-        fft_multi<float2, NX, true, 2>(f, my, ROWS * LD, tw, tp);
+        fft_multi<C, NX, true, 2>(f, my, ROWS * LD, tw, tp);
         if constexpr (THERMAL) {
             hv = heat_slice(g - 1, nz, I.nzh);
             load_vox<NX>(vox, I, hv, y, tp);
         }
-        nl_row<NX, THERMAL>(p, s, I, g - 1, 1, y, tp, vox, hv, THERMAL);
+        nl_row<NX, THERMAL, C>(p, s, I, g - 1, 1, y, tp, vox, hv, THERMAL);
         if (g - 1 == nz - 2) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                I.outP[base + q * TPR] = p[q];
-                I.outS[base + q * TPR] = s[q];
+                const size_t i = base + q * TPR;
+                I.outP[i] = to_store(p[q], i, store_key(2, 0, g));
+                I.outS[i] = to_store(s[q], i, store_key(2, 1, g));
             }
         }
         if (trace) {
+            // same key as the saved output: a traced slice nz-1 is that state, bit for bit
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                I.trP[base + q * TPR] = p[q];
-                I.trS[base + q * TPR] = s[q];
+                const size_t i = base + q * TPR;
+                I.trP[i] = to_store(p[q], i, store_key(2, 0, g));
+                I.trS[i] = to_store(s[q], i, store_key(2, 1, g));
             }
         }
     }
@@ -198,25 +264,28 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
             h = heat_slice(g, nz, I.nzh);
             if (h != hv) load_vox<NX>(vox, I, h, y, tp);
         }
-        nl_row<NX, THERMAL>(p, s, I, g, 0, y, tp, vox, h, false);
-        fft_multi<float2, NX, false, 2>(f, my, ROWS * LD, tw, tp);
+        nl_row<NX, THERMAL, C>(p, s, I, g, 0, y, tp, vox, h, false);
+        fft_multi<C, NX, false, 2>(f, my, ROWS * LD, tw, tp);
     }
+    const uint32_t kp = store_key(0, 0, g), ks = store_key(0, 1, g);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        I.P[base + q * TPR] = p[q];
-        I.S[base + q * TPR] = s[q];
+        const size_t i = base + q * TPR;
+        I.P[i] = to_store(p[q], i, kp);
+        I.S[i] = to_store(s[q], i, ks);
     }
 }
 
 // Column pass of step g: column FFT -> * Prop -> column IFFT on COLS
 // adjacent columns of one field (blockIdx.y: 0 pump, 1 SH).
 // grid (ceil(max_nx/COLS), 2, n_instances), block (COLS, NY/8)
-template <int NY, int COLS>
+template <int NY, int COLS, typename C>
 __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __restrict__ insts, int g,
                                                            const double2* __restrict__ tw) {
+    using R = re_t<C>;
     constexpr int TPC = NY / 8;
     constexpr int LD = FftShape<NY>::LDS;
-    __shared__ float2 lds[COLS * LD];
+    __shared__ C lds[COLS * LD];
     const MixedInst& I = insts[blockIdx.z];
     if (g >= I.nz) return;
     const int nx = I.nx;
@@ -238,23 +307,27 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __re
     float2* F = sh ? I.S : I.P;
     const double2* ex = sh ? I.exS : I.exP;
     const double2* ey = sh ? I.eyS : I.eyP;
-    float2* my = lds + c * LD;
-    float2 v[8];
+    C* my = lds + c * LD;
+    C v[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) v[q] = F[(size_t)(tp + q * TPC) * nx + x];
-    fft_dist<float2, NY, false>(v, my, tw, tp);
-    // Prop = ex[x] ey[y], formed and applied in FP64 and rounded once, so
-    // the propagator carries no fixed FP32 representation error per mode.
+    for (int q = 0; q < 8; q++) v[q] = from_store<C>(F[(size_t)(tp + q * TPC) * nx + x]);
+    fft_dist<C, NY, false>(v, my, tw, tp);
+    // Prop = ex[x] ey[y], formed and applied in FP64 and rounded once to the
+    // arithmetic type, so it carries no fixed representation error per mode.
     const double2 exv = ex[x];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const double2 pr = cmul(exv, ey[tp + q * TPC]);
         const double vr = v[q].x, vi = v[q].y;
-        v[q] = make_float2((float)(vr * pr.x - vi * pr.y), (float)(vr * pr.y + vi * pr.x));
+        v[q] = mk<C>((R)(vr * pr.x - vi * pr.y), (R)(vr * pr.y + vi * pr.x));
     }
-    fft_dist<float2, NY, true>(v, my, tw, tp);
+    fft_dist<C, NY, true>(v, my, tw, tp);
+    const uint32_t key = store_key(1, sh ? 1 : 0, g);
 #pragma unroll
-    for (int q = 0; q < 8; q++) F[(size_t)(tp + q * TPC) * nx + x] = v[q];
+    for (int q = 0; q < 8; q++) {
+        const size_t i = (size_t)(tp + q * TPC) * nx + x;
+        F[i] = to_store(v[q], i, key);
+    }
 }
 
 // ------------------------------------------------------------------
@@ -263,52 +336,40 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __re
 
 // Built-in tile plans (measured on MI355X: rows per workgroup of the row
 // pass, columns per workgroup of the column pass); shg_autotune refines them.
-int rowpass_rows_default(int nx) {
+int rowpass_rows_default(int nx, bool fp64) {
     switch (nx) {
-        case 64: return 8;
-        case 128: return 4;
+        case 64: return fp64 ? 4 : 8;
+        case 128: return fp64 ? 2 : 4;
         case 192: return 2;
-        case 256: return 2;
+        case 256: return fp64 ? 1 : 2;
         case 384: return 1;
         case 512: return 1;
     }
     return 0;
 }
 bool rowpass_rows_supported(int nx, int rows) {
-    return (rows == 1 || rows == 2 || rows == 4 || rows == 8 || rows == 16) && nx / 8 * rows <= 1024;
+    return (rows == 1 || rows == 2 || rows == 4 || rows == 8) && nx / 8 * rows <= 1024;
 }
-int colpass_cols_default(int ny) {
-    switch (ny) {
-        case 64: return 8;
-        case 128: return 8;
-        case 192: return 8;
-        case 256: return 8;
-        case 384: return 8;
-        case 512: return 8;
-    }
-    return 0;
+int colpass_cols_default(int ny, bool fp64) {
+    (void)ny;
+    (void)fp64;
+    return 8;
 }
 bool colpass_cols_supported(int ny, int cols) {
-    return (cols == 2 || cols == 4 || cols == 8 || cols == 16) && ny / 8 * cols <= 1024;
+    return (cols == 4 || cols == 8 || cols == 16) && ny / 8 * cols <= 1024;
 }
 
-template <int NX, bool TH>
+template <int NX, bool TH, typename C>
 static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
                                const double2* tw) {
     dim3 block(NX / 8, rows);
     switch (rows) {
-        case 1: hipLaunchKernelGGL((k_rowpass<NX, TH, 1>), grid, block, 0, st, insts, g, tw); break;
-        case 2: hipLaunchKernelGGL((k_rowpass<NX, TH, 2>), grid, block, 0, st, insts, g, tw); break;
-        case 4: hipLaunchKernelGGL((k_rowpass<NX, TH, 4>), grid, block, 0, st, insts, g, tw); break;
+        case 1: hipLaunchKernelGGL((k_rowpass<NX, TH, 1, C>), grid, block, 0, st, insts, g, tw); break;
+        case 2: hipLaunchKernelGGL((k_rowpass<NX, TH, 2, C>), grid, block, 0, st, insts, g, tw); break;
+        case 4: hipLaunchKernelGGL((k_rowpass<NX, TH, 4, C>), grid, block, 0, st, insts, g, tw); break;
         case 8:
             if constexpr (NX / 8 * 8 <= 1024) {
-                hipLaunchKernelGGL((k_rowpass<NX, TH, 8>), grid, block, 0, st, insts, g, tw);
-                break;
-            }
-            return hipErrorInvalidValue;
-        case 16:
-            if constexpr (NX / 8 * 16 <= 1024) {
-                hipLaunchKernelGGL((k_rowpass<NX, TH, 16>), grid, block, 0, st, insts, g, tw);
+                hipLaunchKernelGGL((k_rowpass<NX, TH, 8, C>), grid, block, 0, st, insts, g, tw);
                 break;
             }
             return hipErrorInvalidValue;
@@ -317,35 +378,39 @@ static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedI
     return hipGetLastError();
 }
 
-hipError_t launch_rowpass(int nx, bool thermal, int rows, dim3 grid, hipStream_t st,
+template <int NX>
+static hipError_t rowpass_n(bool thermal, bool fp64, int rows, dim3 grid, hipStream_t st, const MixedInst* insts,
+                            int g, const double2* tw) {
+    if (fp64)
+        return thermal ? rowpass_rows<NX, true, double2>(rows, grid, st, insts, g, tw)
+                       : rowpass_rows<NX, false, double2>(rows, grid, st, insts, g, tw);
+    return thermal ? rowpass_rows<NX, true, float2>(rows, grid, st, insts, g, tw)
+                   : rowpass_rows<NX, false, float2>(rows, grid, st, insts, g, tw);
+}
+
+hipError_t launch_rowpass(int nx, bool thermal, bool fp64, int rows, dim3 grid, hipStream_t st,
                           const MixedInst* insts, int g, const double2* tw) {
-#define ROWCASE(N)                                                                 \
-    case N:                                                                        \
-        return thermal ? rowpass_rows<N, true>(rows, grid, st, insts, g, tw)       \
-                       : rowpass_rows<N, false>(rows, grid, st, insts, g, tw);
     switch (nx) {
-        ROWCASE(64)
-        ROWCASE(128)
-        ROWCASE(192)
-        ROWCASE(256)
-        ROWCASE(384)
-        ROWCASE(512)
+        case 64: return rowpass_n<64>(thermal, fp64, rows, grid, st, insts, g, tw);
+        case 128: return rowpass_n<128>(thermal, fp64, rows, grid, st, insts, g, tw);
+        case 192: return rowpass_n<192>(thermal, fp64, rows, grid, st, insts, g, tw);
+        case 256: return rowpass_n<256>(thermal, fp64, rows, grid, st, insts, g, tw);
+        case 384: return rowpass_n<384>(thermal, fp64, rows, grid, st, insts, g, tw);
+        case 512: return rowpass_n<512>(thermal, fp64, rows, grid, st, insts, g, tw);
     }
-#undef ROWCASE
     return hipErrorInvalidValue;
 }
 
-template <int NY>
+template <int NY, typename C>
 static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
                                const double2* tw) {
     dim3 block(cols, NY / 8);
     switch (cols) {
-        case 2: hipLaunchKernelGGL((k_colpass<NY, 2>), grid, block, 0, st, insts, g, tw); break;
-        case 4: hipLaunchKernelGGL((k_colpass<NY, 4>), grid, block, 0, st, insts, g, tw); break;
-        case 8: hipLaunchKernelGGL((k_colpass<NY, 8>), grid, block, 0, st, insts, g, tw); break;
+        case 4: hipLaunchKernelGGL((k_colpass<NY, 4, C>), grid, block, 0, st, insts, g, tw); break;
+        case 8: hipLaunchKernelGGL((k_colpass<NY, 8, C>), grid, block, 0, st, insts, g, tw); break;
         case 16:
-            if constexpr (NY / 8 * 16 <= 1024) {
-                hipLaunchKernelGGL((k_colpass<NY, 16>), grid, block, 0, st, insts, g, tw);
+            if constexpr (NY / 8 * 16 <= 1024 && sizeof(C) * 16 * FftShape<NY>::LDS <= 160 * 1024) {
+                hipLaunchKernelGGL((k_colpass<NY, 16, C>), grid, block, 0, st, insts, g, tw);
                 break;
             }
             return hipErrorInvalidValue;
@@ -354,15 +419,22 @@ static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedI
     return hipGetLastError();
 }
 
-hipError_t launch_colpass(int ny, int cols, dim3 grid, hipStream_t st, const MixedInst* insts,
+template <int NY>
+static hipError_t colpass_n(bool fp64, int cols, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
+                            const double2* tw) {
+    return fp64 ? colpass_cols<NY, double2>(cols, grid, st, insts, g, tw)
+                : colpass_cols<NY, float2>(cols, grid, st, insts, g, tw);
+}
+
+hipError_t launch_colpass(int ny, bool fp64, int cols, dim3 grid, hipStream_t st, const MixedInst* insts,
                           int g, const double2* tw) {
     switch (ny) {
-        case 64: return colpass_cols<64>(cols, grid, st, insts, g, tw);
-        case 128: return colpass_cols<128>(cols, grid, st, insts, g, tw);
-        case 192: return colpass_cols<192>(cols, grid, st, insts, g, tw);
-        case 256: return colpass_cols<256>(cols, grid, st, insts, g, tw);
-        case 384: return colpass_cols<384>(cols, grid, st, insts, g, tw);
-        case 512: return colpass_cols<512>(cols, grid, st, insts, g, tw);
+        case 64: return colpass_n<64>(fp64, cols, grid, st, insts, g, tw);
+        case 128: return colpass_n<128>(fp64, cols, grid, st, insts, g, tw);
+        case 192: return colpass_n<192>(fp64, cols, grid, st, insts, g, tw);
+        case 256: return colpass_n<256>(fp64, cols, grid, st, insts, g, tw);
+        case 384: return colpass_n<384>(fp64, cols, grid, st, insts, g, tw);
+        case 512: return colpass_n<512>(fp64, cols, grid, st, insts, g, tw);
     }
     return hipErrorInvalidValue;
 }

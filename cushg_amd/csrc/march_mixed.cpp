@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -29,38 +30,39 @@ struct MixedHost {
     float2 *trP = nullptr, *trS = nullptr;
     int pass = 0;                 // thermo-optic pass being marched
     double2 *exP = nullptr, *exS = nullptr, *eyP = nullptr, *eyS = nullptr;
-    float2* phase = nullptr;
+    double2* phase = nullptr;
     double* pol = nullptr;
     double* slots = nullptr;  // [0] P_in, [1] P_out, [2] P_sh
     HeatState heat;
     MixedInst desc{};
     bool thermal = false;
+    bool fp64 = false;    // SHG_PREC_FP32_STORAGE: FP64 arithmetic in the fused passes
     bool running = true;  // THERMAL: still in the outer loop
     int outer = 0;
     long long steps = 0, sweeps = 0;
     double heat_res = 0;
 };
 
-static const char* rowpass_name(int nx, bool th) {
-    static std::map<std::pair<int, bool>, std::string> names;
-    auto& s = names[{nx, th}];
-    if (s.empty()) s = std::string("rowpass_") + std::to_string(nx) + (th ? "_thermal" : "");
+static const char* rowpass_name(int nx, bool th, bool fp64) {
+    static std::map<std::tuple<int, bool, bool>, std::string> names;
+    auto& s = names[{nx, th, fp64}];
+    if (s.empty()) s = std::string("rowpass_") + std::to_string(nx) + (th ? "_thermal" : "") + (fp64 ? "_f64" : "");
     return s.c_str();
 }
-static const char* colpass_name(int ny) {
-    static std::map<int, std::string> names;
-    auto& s = names[ny];
-    if (s.empty()) s = "colpass_" + std::to_string(ny);
+static const char* colpass_name(int ny, bool fp64) {
+    static std::map<std::pair<int, bool>, std::string> names;
+    auto& s = names[{ny, fp64}];
+    if (s.empty()) s = "colpass_" + std::to_string(ny) + (fp64 ? "_f64" : "");
     return s.c_str();
 }
 
-int plan_rows(shg_handle* h, int nx) {
-    auto it = h->rows_plan.find(nx);
-    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx);
+int plan_rows(shg_handle* h, int nx, bool fp64) {
+    auto it = h->rows_plan.find({nx, fp64});
+    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx, fp64);
 }
-int plan_cols(shg_handle* h, int ny) {
-    auto it = h->cols_plan.find(ny);
-    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_default(ny);
+int plan_cols(shg_handle* h, int ny, bool fp64) {
+    auto it = h->cols_plan.find({ny, fp64});
+    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_default(ny, fp64);
 }
 
 // Host-side FP64 tables of one instance at centre temperature Tc:
@@ -69,7 +71,7 @@ int plan_cols(shg_handle* h, int ny) {
 // into the x factor), per-slice QPM phases, couplings and RK4 constants.
 static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>& ex_s,
                         std::vector<double2>& ey_p, std::vector<double2>& ey_s,
-                        std::vector<float2>& phase) {
+                        std::vector<double2>& phase) {
     const shg_params* p = m.p;
     const int nx = m.nx, ny = m.ny, nz = m.nz;
     const double lp = p->lambda_p_um, ls = 0.5 * lp;
@@ -104,38 +106,36 @@ static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<do
     double off[2][3] = {{0.0, 0.25, 0.5}, {0.0, 0.25, 0.5}};
     if (p->scheme == SHG_SCHEME_STRANG) {
         const double hh = 0.5 * m.dz;
-        d.c1 = (float)(0.5 * hh);
-        d.c2 = (float)(0.5 * hh);
-        d.c3 = (float)hh;
-        d.w = (float)(hh / 6.0);
+        d.c1 = 0.5 * hh;
+        d.c2 = 0.5 * hh;
+        d.c3 = hh;
+        d.w = hh / 6.0;
         kmul = 2.0;
         off[1][0] = 0.5;
         off[1][1] = 0.75;
         off[1][2] = 1.0;
     } else {
-        d.c1 = 0.5f;
-        d.c2 = 0.5f;
-        d.c3 = 1.0f;
-        d.w = (float)(m.dz / 6.0);
+        d.c1 = 0.5;
+        d.c2 = 0.5;
+        d.c3 = 1.0;
+        d.w = m.dz / 6.0;
     }
     for (int c = 0; c < 2; c++)
-        for (int i = 0; i < 3; i++) d.off[c][i] = (float)off[c][i];
+        for (int i = 0; i < 3; i++) d.off[c][i] = off[c][i];
     const double dQ = 2.0 * p->deff_um_per_V / kPi;
-    d.kp = (float)(kmul * kPi * dQ / (np * lp));
-    d.ks = (float)(kmul * 0.5 * kPi * dQ / (ns * ls));
-    d.bfac = (float)(p->beta_s * 0.5 * kC * kEps0 * ns);
-    d.kp0 = (float)(kmul * kPi * dQ / lp);
-    d.ks0 = (float)(kmul * 0.5 * kPi * dQ / ls);
-    d.bf0 = (float)(p->beta_s * 0.5 * kC * kEps0);
-    d.ap_h = (float)(0.5 * p->alpha_p);
-    d.as_h = (float)(0.5 * p->alpha_s);
-    d.hce = (float)(0.5 * kC * kEps0);
-    d.alpha_p = (float)p->alpha_p;
-    d.alpha_s = (float)p->alpha_s;
-    d.beta_s = (float)p->beta_s;
+    d.kp = kmul * kPi * dQ / (np * lp);
+    d.ks = kmul * 0.5 * kPi * dQ / (ns * ls);
+    d.bfac = p->beta_s * 0.5 * kC * kEps0 * ns;
+    d.kp0 = kmul * kPi * dQ / lp;
+    d.ks0 = kmul * 0.5 * kPi * dQ / ls;
+    d.bf0 = p->beta_s * 0.5 * kC * kEps0;
+    d.ap_h = 0.5 * p->alpha_p;
+    d.as_h = 0.5 * p->alpha_s;
+    d.hce = 0.5 * kC * kEps0;
+    d.alpha_p = p->alpha_p;
+    d.alpha_s = p->alpha_s;
+    d.beta_s = p->beta_s;
     d.crystal = p->crystal;
-    d.lp = (float)lp;
-    d.ls = (float)ls;
     d.dz = m.dz;
     d.nx = nx;
     d.ny = ny;
@@ -150,7 +150,7 @@ static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<do
             for (int c = 0; c < 2; c++)
                 for (int i = 0; i < 3; i++) {
                     const double ph = ((double)s + off[c][i]) * m.dz * dk + pol;
-                    phase[(size_t)(s * 2 + c) * 3 + i] = make_float2((float)std::cos(ph), (float)std::sin(ph));
+                    phase[(size_t)(s * 2 + c) * 3 + i] = make_double2(std::cos(ph), std::sin(ph));
                 }
         }
     } else {
@@ -162,14 +162,14 @@ static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<do
 // zero SH field, and reduce the initial pump power.
 static int setup_fields(shg_handle* h, MixedHost& m) {
     std::vector<double2> exp_, exs, eyp, eys;
-    std::vector<float2> ph;
+    std::vector<double2> ph;
     host_tables(m, exp_, exs, eyp, eys, ph);
     SHG_HIP(hipMemcpyAsync(m.exP, exp_.data(), sizeof(double2) * m.nx, hipMemcpyHostToDevice, h->stream));
     SHG_HIP(hipMemcpyAsync(m.exS, exs.data(), sizeof(double2) * m.nx, hipMemcpyHostToDevice, h->stream));
     SHG_HIP(hipMemcpyAsync(m.eyP, eyp.data(), sizeof(double2) * m.ny, hipMemcpyHostToDevice, h->stream));
     SHG_HIP(hipMemcpyAsync(m.eyS, eys.data(), sizeof(double2) * m.ny, hipMemcpyHostToDevice, h->stream));
     if (!ph.empty())
-        SHG_HIP(hipMemcpyAsync(m.phase, ph.data(), sizeof(float2) * ph.size(), hipMemcpyHostToDevice, h->stream));
+        SHG_HIP(hipMemcpyAsync(m.phase, ph.data(), sizeof(double2) * ph.size(), hipMemcpyHostToDevice, h->stream));
     // the host vectors die at return: make the copies complete first
     SHG_HIP(hipStreamSynchronize(h->stream));
     const shg_params* p = m.p;
@@ -211,19 +211,23 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
     struct Group {
         int key_n;
         bool thermal;
+        bool fp64;
         std::vector<MixedHost*> m;
         MixedInst* d = nullptr;
         const double2* tw = nullptr;
     };
-    std::map<std::pair<int, bool>, Group> rowg;
-    std::map<int, Group> colg;
+    std::map<std::tuple<int, bool, bool>, Group> rowg;
+    std::map<std::pair<int, bool>, Group> colg;
     for (auto* m : act) {
-        auto& rg = rowg[{m->nx, m->thermal}];
+        auto& rg = rowg[{m->nx, m->thermal, m->fp64}];
         rg.key_n = m->nx;
         rg.thermal = m->thermal;
+        rg.fp64 = m->fp64;
         rg.m.push_back(m);
-        auto& cg = colg[m->ny];
+        auto& cg = colg[{m->ny, m->fp64}];
         cg.key_n = m->ny;
+        cg.thermal = false;
+        cg.fp64 = m->fp64;
         cg.m.push_back(m);
     }
     auto by_nz = [](const MixedHost* a, const MixedHost* b) { return a->nz > b->nz; };
@@ -263,10 +267,10 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
                 maxny = std::max(maxny, m->ny);
             }
             if (!cnt) continue;
-            const int rows = plan_rows(h, gr.key_n);
+            const int rows = plan_rows(h, gr.key_n, gr.fp64);
             dim3 grid((maxny + rows - 1) / rows, 1, cnt);
-            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal));
-            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, rows, grid, h->stream, gr.d, g, gr.tw));
+            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.fp64));
+            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.fp64, rows, grid, h->stream, gr.d, g, gr.tw));
         }
         for (auto* m : act)
             if (m->trace && m->trace->wants(g, m->nz))
@@ -281,10 +285,10 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
                 maxnx = std::max(maxnx, m->nx);
             }
             if (!cnt) continue;
-            const int cols = plan_cols(h, gr.key_n);
+            const int cols = plan_cols(h, gr.key_n, gr.fp64);
             dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
-            ProfScope ps(h, colpass_name(gr.key_n));
-            SHG_HIP(launch_colpass(gr.key_n, cols, grid, h->stream, gr.d, g, gr.tw));
+            ProfScope ps(h, colpass_name(gr.key_n, gr.fp64));
+            SHG_HIP(launch_colpass(gr.key_n, gr.fp64, cols, grid, h->stream, gr.d, g, gr.tw));
         }
     }
     // descriptors' host copies must outlive the async copies
@@ -296,7 +300,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
 static size_t mixed_bytes(const MixedHost& m) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t b = 4 * al(sizeof(float2) * m.N2) + 2 * al(sizeof(double2) * m.nx) +
-               2 * al(sizeof(double2) * m.ny) + al(sizeof(float2) * (size_t)m.nz * 6) + al(sizeof(double) * 4) +
+               2 * al(sizeof(double2) * m.ny) + al(sizeof(double2) * (size_t)m.nz * 6) + al(sizeof(double) * 4) +
                2 * al(sizeof(MixedInst));
     if (m.thermal) b += heat_bytes(m.p);
     if (m.trace) b += 2 * al(sizeof(float2) * m.N2);
@@ -339,6 +343,7 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         m.dy = m.p->ly_um / (m.ny - 1);
         m.dz = m.p->lz_um / (m.nz - 1);
         m.thermal = m.p->thermal != 0;
+        m.fp64 = m.p->precision == SHG_PREC_FP32_STORAGE;
         m.Tc = m.p->T_C;
         total += mixed_bytes(m);
     }
@@ -353,7 +358,7 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         m.exS = h->arena.carve<double2>(m.nx);
         m.eyP = h->arena.carve<double2>(m.ny);
         m.eyS = h->arena.carve<double2>(m.ny);
-        m.phase = h->arena.carve<float2>((size_t)m.nz * 6);
+        m.phase = h->arena.carve<double2>((size_t)m.nz * 6);
         if (m.trace) {
             m.trP = h->arena.carve<float2>(m.N2);
             m.trS = h->arena.carve<float2>(m.N2);

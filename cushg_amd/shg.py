@@ -78,8 +78,8 @@ def lib():
     L.shg_last_kernel_times.argtypes = [H, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int32), C.c_int32]
     L.shg_handle_set_profiling.argtypes = [H, C.c_int32]
-    L.shg_set_plan.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
-    L.shg_autotune.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
+    L.shg_set_plan.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.shg_autotune.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     _lib = L
     return L
 
@@ -210,11 +210,11 @@ class Handle:
         n = lib().shg_last_kernel_times(self._h, names, ms, nl, cap)
         return {names[i].decode(): (ms[i], nl[i]) for i in range(n)}
 
-    def set_plan(self, n: int, rows: int = 0, cols: int = 0):
-        _check(lib().shg_set_plan(self._h, n, rows, cols))
+    def set_plan(self, n: int, rows: int = 0, cols: int = 0, precision: int = abi.PREC_FP32_MIXED):
+        _check(lib().shg_set_plan(self._h, n, precision, rows, cols))
 
-    def autotune(self, n: int, batch: int = 1, nz: int = 64):
-        _check(lib().shg_autotune(self._h, n, batch, nz))
+    def autotune(self, n: int, precision: int = abi.PREC_FP32_MIXED, batch: int = 1, nz: int = 64):
+        _check(lib().shg_autotune(self._h, n, precision, batch, nz))
 
 
 def save_matrix_complex(A: np.ndarray, name: str, out_dir: str = ".") -> None:

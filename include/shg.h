@@ -64,10 +64,17 @@ enum { SHG_CRYSTAL_MGO_SPPLT = 0, SHG_CRYSTAL_MGO_PPLN = 1 };
  * NL(dz/2)-D(dz)-NL(dz/2) with standard RK4 (used for closed-form checks). */
 enum { SHG_SCHEME_REFERENCE = 0, SHG_SCHEME_STRANG = 1 };
 
-/* Arithmetic form on the GPU.  FP32_MIXED: FP32 fields, FFTs and RK4 with
- * FP64 phase (dk*z), FP64 reductions and FP64 heat residual.  FP64_PLAIN:
- * FP64 everywhere, unfused kernels in the reference's op order. */
-enum { SHG_PREC_FP32_MIXED = 0, SHG_PREC_FP64_PLAIN = 1 };
+/* Arithmetic form on the GPU.
+ *   FP32_MIXED:   FP32 fields, FFT butterflies and RK4; FP64 phase (dk*z),
+ *                 twiddle and propagator products, reductions and heat
+ *                 residual.
+ *   FP64_PLAIN:   FP64 everywhere, unfused kernels in the reference's op
+ *                 order (one instance at a time).
+ *   FP32_STORAGE: the fused, batched march of FP32_MIXED with the same FP32
+ *                 fields in device memory (same HBM bytes), but every
+ *                 operation between a load and a store -- FFTs, propagator,
+ *                 phase, RK4 -- in FP64 registers. */
+enum { SHG_PREC_FP32_MIXED = 0, SHG_PREC_FP64_PLAIN = 1, SHG_PREC_FP32_STORAGE = 2 };
 
 /* Heat solver for THERMAL runs.  JACOBI_REFERENCE: the reference's loop
  * (src/headers/Solver.h:332-356): Jacobi sweeps, convergence tested every
@@ -240,14 +247,15 @@ SHG_API int shg_last_kernel_times(shg_handle* h, const char** names, double* ms,
  * stream; read the totals with shg_last_kernel_times.  Resets the totals. */
 SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
 
-/* Launch plan of the fused passes for transform length n: rows per
- * workgroup of the row pass and columns per workgroup of the column pass
- * (0 keeps the built-in default).  Used by the autotuner. */
-SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t rows_per_wg, int32_t cols_per_wg);
+/* Launch plan of the fused passes for transform length n and precision
+ * (SHG_PREC_FP32_MIXED or SHG_PREC_FP32_STORAGE): rows per workgroup of
+ * the row pass and columns per workgroup of the column pass (0 keeps the
+ * current choice).  Used by the autotuner. */
+SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t rows_per_wg, int32_t cols_per_wg);
 
 /* Time the candidate plans for transform length n on a synthetic march of
  * `batch` instances and keep the fastest on the handle. */
-SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t batch, int32_t nz);
+SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int32_t nz);
 
 #ifdef __cplusplus
 }

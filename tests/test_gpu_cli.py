@@ -19,7 +19,7 @@ def rel_l2(a, b):
 
 
 def small(precision=abi.PREC_FP32_MIXED):
-    p = configs.config1()
+    p = configs.spplt64()
     p.nz = 24
     p.alpha_p, p.alpha_s, p.beta_s = 0.17e-6, 1.57e-6, 5e-5
     p.precision = precision
@@ -38,7 +38,8 @@ def oracle_slices(p):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,tol", [(abi.PREC_FP32_MIXED, 2e-5), (abi.PREC_FP64_PLAIN, 1e-11)])
+@pytest.mark.parametrize("precision,tol", [(abi.PREC_FP32_MIXED, 2e-5), (abi.PREC_FP32_STORAGE, 2e-6),
+                                           (abi.PREC_FP64_PLAIN, 1e-11)])
 def test_trace_slices_match_oracle_march(handle, precision, tol):
     p = small(precision)
     print(f"case trace precision={precision}", flush=True)
@@ -61,7 +62,7 @@ def test_trace_slices_match_oracle_march(handle, precision, tol):
 
 @pytest.mark.gpu
 def test_trace_stride_passes_and_cancel(handle):
-    p = configs.scaled(configs.config2(), n=64, nz=40, nz_heat=10)
+    p = configs.scaled(configs.spplt256_thermal(), n=64, nz=40, nz_heat=10)
     p.lx_um = p.ly_um = 400.0
     p.alpha_p, p.alpha_s = 2e-6, 2e-5
     seen = []

@@ -1,8 +1,10 @@
-"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle.
+"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle,
+for both fused FP32-field forms.
 
-Tolerance (north_star): eta relative error <= 1e-5 against FP64.  The
-fields themselves are FP32 (relative L2 error ~1e-6 expected); they are
-held to 2e-5.
+Tolerances: eta relative error <= 1e-5 against FP64 (the accuracy bar of
+the FP32 batched form).  The fields are stored in FP32: relative L2 error
+<= 2e-5 with FP32 arithmetic (SHG_PREC_FP32_MIXED), <= 2e-6 with FP64
+arithmetic (SHG_PREC_FP32_STORAGE, where only the stores round).
 """
 import numpy as np
 import pytest
@@ -23,8 +25,21 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def _check(p, handle, eta_tol=ETA_TOL, field_tol=FIELD_TOL):
-    print(f"case nx={p.nx} ny={p.ny} nz={p.nz} scheme={p.scheme} thermal={p.thermal}", flush=True)
+FORMS = [pytest.param(abi.PREC_FP32_MIXED, id="mixed"), pytest.param(abi.PREC_FP32_STORAGE, id="storage")]
+
+
+@pytest.fixture(params=FORMS)
+def form(request):
+    return request.param
+
+
+def _check(p, handle, form=abi.PREC_FP32_MIXED, eta_tol=ETA_TOL, field_tol=None):
+    p = p.copy()
+    p.precision = form
+    if field_tol is None:
+        field_tol = FIELD_TOL if form == abi.PREC_FP32_MIXED else 2e-6
+    print(f"case nx={p.nx} ny={p.ny} nz={p.nz} scheme={p.scheme} thermal={p.thermal} precision={form}",
+          flush=True)
     ro, o = O.run(p)
     rg, g = handle.run(p)
     assert rg.status == 0
@@ -42,60 +57,62 @@ def _check(p, handle, eta_tol=ETA_TOL, field_tol=FIELD_TOL):
 
 
 @pytest.mark.gpu
-def test_config1_parity(handle):
-    """Config 1: 64x64, 100 slices, 30 mm MgO:sPPLT, 1 W, 30 um, 55.66 C."""
-    _check(configs.config1(), handle)
+def test_spplt64_parity(handle, form):
+    """64x64, 100 slices, 30 mm MgO:sPPLT, 1 W, 30 um, 55.66 C."""
+    _check(configs.spplt64(), handle, form)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n", [64, 128, 192, 256, 384, 512])
-def test_sizes_reference_defaults(handle, n):
+def test_sizes_reference_defaults(handle, n, form):
     """Every transform length with the reference's absorption and 2PA on."""
     p = abi.default_params()
     p.nx = p.ny = n
     p.nz = 24
     p.lx_um = p.ly_um = 800.0
     p.power_W = 20.0
-    _check(p, handle)
+    _check(p, handle, form)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nx,ny", [(128, 64), (192, 256), (512, 128)])
-def test_rectangular_grids(handle, nx, ny):
-    p = configs.config1()
+def test_rectangular_grids(handle, nx, ny, form):
+    p = configs.spplt64()
     p.nx, p.ny, p.nz = nx, ny, 16
     p.lx_um, p.ly_um = 1200.0, 700.0
-    _check(p, handle)
+    _check(p, handle, form)
 
 
 @pytest.mark.gpu
-def test_strang_scheme(handle):
-    p = configs.config1()
+def test_strang_scheme(handle, form):
+    p = configs.spplt64()
     p.scheme = abi.SCHEME_STRANG
     p.alpha_p, p.alpha_s, p.beta_s = 0.17e-6, 1.57e-6, 5e-5
-    _check(p, handle)
+    _check(p, handle, form)
 
 
 @pytest.mark.gpu
-def test_walkoff_and_offcentre_focus(handle):
-    p = configs.config1()
+def test_walkoff_and_offcentre_focus(handle, form):
+    p = configs.spplt64()
     p.rho_p, p.rho_s = 0.0, 0.01
     p.focal_um = 0.2 * p.lz_um
     p.nz = 40
-    _check(p, handle)
+    _check(p, handle, form)
 
 
 @pytest.mark.gpu
-def test_poling_phase(handle):
-    p = configs.config4(nz=60, n=64)
-    _check(p, handle)
+def test_poling_phase(handle, form):
+    p = configs.adjoint256(nz=60, n=64)
+    _check(p, handle, form)
 
 
 @pytest.mark.gpu
 def test_batch_equals_sequential(handle):
-    """Uneven batch (different grids and step counts) in one execution gives
-    the same bits as running each instance alone."""
-    ps = configs.config3(seed=3, n=5, scale_steps=0.02)
+    """Uneven batch (different grids, step counts and both fused forms) in
+    one execution gives the same bits as running each instance alone."""
+    ps = configs.sweep_batch(seed=3, n=6, scale_steps=0.02)
+    for i, p in enumerate(ps):
+        p.precision = abi.PREC_FP32_STORAGE if i % 2 else abi.PREC_FP32_MIXED
     rc, rb, ob = handle.run_batch(ps, fields=True)
     assert rc == 0
     for p, r, o in zip(ps, rb, ob):
@@ -107,7 +124,7 @@ def test_batch_equals_sequential(handle):
 
 @pytest.mark.gpu
 def test_invalid_size_is_einval(handle):
-    p = configs.config1()
+    p = configs.spplt64()
     p.nx = 100
     with pytest.raises(S.ShgError) as ei:
         handle.run(p)

@@ -18,7 +18,7 @@ def rel(a, b):
 
 
 def small_thermal(n=64, nz=40, nzh=12, solver=abi.HEAT_DIRECT):
-    p = configs.config2()
+    p = configs.spplt256_thermal()
     p.nx = p.ny = n
     p.nz = nz
     p.nz_heat = nzh
@@ -37,7 +37,7 @@ def small_thermal(n=64, nz=40, nzh=12, solver=abi.HEAT_DIRECT):
 def test_thermal_uniform_equals_plain(handle):
     """No heating and T_inf = T_oven = T: the THERMAL march (per-voxel n(T)
     and running dk integral) reproduces the plain march."""
-    p = configs.config1()
+    p = configs.spplt64()
     p.nz = 60
     p.T_inf_C = p.T_peltier_C = p.T_C
     r0, o0 = handle.run(p)
@@ -52,11 +52,16 @@ def test_thermal_uniform_equals_plain(handle):
     assert abs(r1.peak_dT_K) < 1e-9
 
 
+FORMS = [pytest.param(abi.PREC_FP32_MIXED, id="mixed"), pytest.param(abi.PREC_FP32_STORAGE, id="storage")]
+
+
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("nzh", [40, 12])
-def test_thermal_direct_vs_oracle(handle, nzh):
+def test_thermal_direct_vs_oracle(handle, nzh, form):
     p = small_thermal(nzh=nzh)
-    print(f"case thermal direct nx={p.nx} nz={p.nz} nzh={nzh}", flush=True)
+    p.precision = form
+    print(f"case thermal direct nx={p.nx} nz={p.nz} nzh={nzh} precision={form}", flush=True)
     ro, o = O.run(p, temperature=True)
     rg, g = handle.run(p, temperature=True)
     print(f"  eta gpu={rg.eta:.10e} cpu={ro.eta:.10e} rel={rel(rg.eta, ro.eta):.2e} "
@@ -92,14 +97,18 @@ def test_thermal_jacobi_reference_loop_vs_oracle(handle):
 
 
 @pytest.mark.gpu
-def test_config2_scaled_vs_oracle(handle):
-    """Config 2 physics (50 mm, 10 W, 40 um, oven 40 C) at 128x128, 200 z-steps,
-    heat 128x128x25."""
-    p = configs.scaled(configs.config2(), n=128, nz=200, nz_heat=25)
+@pytest.mark.parametrize("form", FORMS)
+def test_spplt256_thermal_scaled_vs_oracle(handle, form):
+    """The 256x256 THERMAL sPPLT physics (50 mm, 10 W, 40 um, oven 40 C) at
+    128x128, 200 z-steps, heat 128x128x25; FP64 arithmetic holds eta to
+    2e-6."""
+    p = configs.scaled(configs.spplt256_thermal(), n=128, nz=200, nz_heat=25)
+    p.precision = form
+    tol = 1e-5 if form == abi.PREC_FP32_MIXED else 2e-6
     ro, o = O.run(p, temperature=True)
     rg, g = handle.run(p, temperature=True)
     print(f"  eta gpu={rg.eta:.10e} cpu={ro.eta:.10e} rel={rel(rg.eta, ro.eta):.2e} "
           f"dT {rg.peak_dT_K:.6e}/{ro.peak_dT_K:.6e} outer {rg.outer_iters}/{ro.outer_iters}")
     assert rg.outer_iters == ro.outer_iters
-    assert rel(rg.eta, ro.eta) <= 1e-5
-    assert rel(rg.peak_dT_K, ro.peak_dT_K) <= 1e-5
+    assert rel(rg.eta, ro.eta) <= tol
+    assert rel(rg.peak_dT_K, ro.peak_dT_K) <= tol

@@ -101,7 +101,7 @@ def test_dk_zero_near_56p92C():
 
 
 def test_qpm_period_helper_phase_matches():
-    p = configs.config2()
+    p = configs.spplt256_thermal()
     assert abs(O.delta_k(p, 40.0)) < 1e-14
 
 
@@ -135,7 +135,7 @@ def _gauss(p, Tc, zrel):
 
 
 def test_pump_init_closed_form_and_power():
-    p = configs.config1()
+    p = configs.spplt64()
     p.nx = p.ny = 128
     A = O.pump_init(p, p.T_C)
     assert np.abs(A - _gauss(p, p.T_C, 0.0)).max() < 1e-12 * np.abs(A).max()
@@ -146,7 +146,7 @@ def test_pump_init_closed_form_and_power():
 def test_free_space_gaussian_matches_analytic():
     """No coupling, no loss: the march is pure diffraction; the saved output
     (after nz-1 slices = the crystal length) is the analytic beam there."""
-    p = configs.config1()
+    p = configs.spplt64()
     p.nx = p.ny = 128
     p.deff_um_per_V = 0.0
     p.nz = 30
@@ -177,7 +177,7 @@ def _tanh_closed_form(p, P0, z, kmul):
 
 
 def _pw_params(nz):
-    p = configs.config1()
+    p = configs.spplt64()
     p.alpha_p = p.alpha_s = p.beta_s = 0.0
     p.nx = p.ny = 8
     p.nz = nz
@@ -217,7 +217,7 @@ def test_reference_scheme_first_order_to_tanh():
 
 def test_strang_march_conserves_power():
     """Manley-Rowe: with alpha = beta = 0, n_p|P|^2 + n_s|S|^2 is invariant."""
-    p = configs.config1()
+    p = configs.spplt64()
     p.scheme = abi.SCHEME_STRANG
     p.power_W = 50.0
     p.nz = 60
@@ -276,7 +276,7 @@ def test_jacobi_iteration_converges_to_direct(heatgold):
 def test_thermal_uniform_equals_plain():
     """T_inf = T_oven = T and no absorption: T stays uniform, the running dk
     integral reduces to z*dk and the THERMAL march equals the plain one."""
-    p = configs.config1()
+    p = configs.spplt64()
     p.nx = p.ny = 32
     p.nz = 24
     p.T_inf_C = p.T_peltier_C = p.T_C
@@ -296,7 +296,7 @@ def test_thermal_uniform_equals_plain():
 def test_thermal_jacobi_reference_loop_vs_direct():
     """With a tight tolerance the reference Jacobi loop reaches the direct
     solver's steady state, so the thermo-optic results agree."""
-    p = configs.config2()
+    p = configs.spplt256_thermal()
     p.nx = p.ny = 16
     p.nz = 12
     p.nz_heat = 6
