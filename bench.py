@@ -155,17 +155,21 @@ def cpu_baseline(ps, budget_s: float):
     if p.thermal:
         p.nz_heat = min(p.nz_heat if p.nz_heat > 0 else p.nz, 16)
     threads = oracle.get_threads()
-    q = p.copy()
-    q.nz = 16
-    t = time.perf_counter()
-    r, _ = oracle.run(q, fields=False)
-    dt = max(time.perf_counter() - t, 1e-3)
-    nz = int(max(16, min(p.nz, q.nz * budget_s / dt)))
-    q = p.copy()
-    q.nz = nz
-    t = time.perf_counter()
-    r, _ = oracle.run(q, fields=False)
-    dt = time.perf_counter() - t
+
+    def timed(nz):
+        q = p.copy()
+        q.nz = nz
+        t = time.perf_counter()
+        r, _ = oracle.run(q, fields=False)
+        return q, r, time.perf_counter() - t
+
+    # fixed cost (heat solve, set-up) and cost per z-step from two short runs
+    _, _, d1 = timed(16)
+    _, _, d2 = timed(48)
+    per = max((d2 - d1) / 32, 1e-4)
+    fixed = max(d1 - 16 * per, 0.0)
+    nz = int(max(16, min(p.nz, (budget_s - fixed) / per)))
+    q, r, dt = timed(nz)
     return {"value": round(q.nx * q.ny * r.steps_done / dt, 1), "unit": UNIT,
             "cores": threads, "kind": "port",
             "sample": f"first instance, {q.nx}x{q.ny}, one thermo-optic pass of {r.steps_done} z-steps"

@@ -129,4 +129,33 @@ __device__ __forceinline__ double2 expi_fast64(double ph) {
     return make_double2((q == 1 || q == 2) ? -re : re, (q >= 2) ? -im : im);
 }
 
+// exp(i t) for a small angle: the Taylor series without range reduction
+// when |t| <= 1/8 -- FP64 through t^9 / t^10, FP32 through t^5 / t^6
+// (truncation error below 1e-17 and 1e-10) -- else the reduced forms above.
+template <typename C> __device__ __forceinline__ C expi_small(double t);
+template <> __device__ __forceinline__ double2 expi_small<double2>(double t) {
+    if (!(fabs(t) <= 0.125)) return expi_fast64(t);
+    const double x2 = t * t;
+    double s = 1.0 / 362880.0;
+    s = s * -x2 + 1.0 / 5040.0;
+    s = s * -x2 + 1.0 / 120.0;
+    s = s * -x2 + 1.0 / 6.0;
+    s = t * (1.0 - x2 * s);
+    double c = 1.0 / 3628800.0;
+    c = c * -x2 + 1.0 / 40320.0;
+    c = c * -x2 + 1.0 / 720.0;
+    c = c * -x2 + 1.0 / 24.0;
+    c = c * -x2 + 0.5;
+    c = 1.0 - x2 * c;
+    return make_double2(c, s);
+}
+template <> __device__ __forceinline__ float2 expi_small<float2>(double t) {
+    if (!(fabs(t) <= 0.125)) return expi_fast(t);
+    const float a = (float)t;
+    const float x2 = a * a;
+    const float s = a * (1.0f - x2 * (1.0f / 6.0f - x2 * (1.0f / 120.0f)));
+    const float c = 1.0f - x2 * (0.5f - x2 * (1.0f / 24.0f - x2 * (1.0f / 720.0f)));
+    return make_float2(c, s);
+}
+
 }  // namespace shg

@@ -57,9 +57,12 @@ __device__ __forceinline__ uint32_t store_key(int site, int field, int g) {
     return mix32(0x9e3779b9U * (uint32_t)(g + 1) ^ (uint32_t)(site << 28 | field << 27));
 }
 __device__ __forceinline__ float2 to_store(float2 v, size_t, uint32_t) { return v; }
+// one hash per complex value: 16 random bits per component, placed at the
+// top of the 29 dropped bits (the 13 bits below them truncate: a bias of at
+// most 2^-16 ulp toward zero per store)
 __device__ __forceinline__ float2 to_store(double2 v, size_t idx, uint32_t key) {
-    const uint32_t k = key ^ (uint32_t)(idx * 2);
-    return make_float2(sr_f32(v.x, mix32(k)), sr_f32(v.y, mix32(k + 1)));
+    const uint32_t r = mix32(key ^ (uint32_t)idx);
+    return make_float2(sr_f32(v.x, r & 0xffff0000U), sr_f32(v.y, r << 16));
 }
 
 template <typename C> __device__ __forceinline__ C expi(double ph);
@@ -120,73 +123,117 @@ __device__ __forceinline__ int heat_count(int h, int nz, int nzh) {
     return (int)(a - b);
 }
 
-// Heat-voxel values a THERMAL R call reads at its 8 points of a row: the
-// refractive indices (n_p, n_s)(T) and the running-dk base and slope.
-struct VoxRow {
-    float2 n[8];
-    double dkb[8], dk[8];
-};
-
-template <int NX>
-__device__ __forceinline__ void load_vox(VoxRow& v, const MixedInst& I, int h, int y, int tp) {
-    constexpr int TPR = NX / 8;
-    const size_t rowoff = (size_t)h * I.nx * I.ny + (size_t)y * I.nx;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int x = tp + q * TPR;
-        v.n[q] = I.nn[rowoff + x];
-        v.dkb[q] = I.dkbase[rowoff + x];
-        v.dk[q] = I.dk[rowoff + x];
-    }
-}
-
-// R call `call` (0 = first half of the slice, 1 = second half) of slice
-// `step` on the 8 points x = tp + q*TPR of row y.  With THERMAL the values
-// of the slice's heat voxels come in `vox` (heat slice h of `step`) and the
-// heat source of the slice is deposited after the second call.
+// Nonlinear work of the row pass at global step g on the 8 points
+// x = tp + q*TPR of row y: R call 2 of slice g-1 (if g > 0; with THERMAL
+// the heat source of slice g-1 is deposited after it; the state is then
+// saved and traced as asked), then R call 1 of slice g (if g < nz).  Both
+// calls run point by point, so a point's heat-voxel values are loaded once
+// and the second call's phase factor is the first's times a small-angle
+// factor exp(i (phi_g - phi_{g-1})).
 template <int NX, bool THERMAL, typename C>
-__device__ __forceinline__ void nl_row(C (&p)[8], C (&s)[8], const MixedInst& I, int step, int call, int y,
-                                       int tp, const VoxRow& vox, int h, bool deposit_q) {
+__device__ __forceinline__ void nl_pair(C (&p)[8], C (&s)[8], const MixedInst& I, int g, int y, int tp,
+                                        size_t base, bool trace) {
     using R = re_t<C>;
     constexpr int TPR = NX / 8;
+    const int nz = I.nz;
+    const bool c1 = g > 0, c0 = g < nz;
+    const bool save = c1 && g - 1 == nz - 2;
     const RkConst<R> k{(R)I.c1, (R)I.c2, (R)I.c3, (R)I.w, (R)I.ap_h, (R)I.as_h};
+    // saved output and trace share a key: a traced slice nz-1 is that state, bit for bit
+    const uint32_t kop = store_key(2, 0, g), kos = store_key(2, 1, g);
+    auto emit = [&](int q) {
+        const size_t i = base + q * TPR;
+        if (save) {
+            I.outP[i] = to_store(p[q], i, kop);
+            I.outS[i] = to_store(s[q], i, kos);
+        }
+        if (trace) {
+            I.trP[i] = to_store(p[q], i, kop);
+            I.trS[i] = to_store(s[q], i, kos);
+        }
+    };
     if constexpr (!THERMAL) {
-        const double2* ph = I.phase + (size_t)(step * 2 + call) * 3;
-        const C e[3] = {mk<C>((R)ph[0].x, (R)ph[0].y), mk<C>((R)ph[1].x, (R)ph[1].y),
-                        mk<C>((R)ph[2].x, (R)ph[2].y)};
+        // phase table: exp(i phi) of (slice, call, offset), FP64
+        C e1[3], e0[3];
+        const double2* t1 = I.phase + (size_t)(max(g - 1, 0) * 2 + 1) * 3;
+        const double2* t0 = I.phase + (size_t)(min(g, nz - 1) * 2) * 3;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            e1[j] = c1 ? mk<C>((R)t1[j].x, (R)t1[j].y) : C{};
+            e0[j] = c0 ? mk<C>((R)t0[j].x, (R)t0[j].y) : C{};
+        }
         const R kp = (R)I.kp, ks = (R)I.ks, bfac = (R)I.bfac;
 #pragma unroll
-        for (int q = 0; q < 8; q++) rk4_point(p[q], s[q], e, kp, ks, bfac, k);
+        for (int q = 0; q < 8; q++) {
+            if (c1) {
+                rk4_point(p[q], s[q], e1, kp, ks, bfac, k);
+                emit(q);
+            }
+            if (c0) rk4_point(p[q], s[q], e0, kp, ks, bfac, k);
+        }
     } else {
-        const double pol = I.poling ? I.poling[step] : 0.0;
-        const double zsc = I.dz / ((double)step + 1.0);
-        const double o0 = (double)step + I.off[call][0];
-        float* Qr = I.Q + (size_t)h * I.nx * I.ny + (size_t)y * I.nx;
-        const bool first = step == 0 || heat_slice(step - 1, I.nz, I.nzh) != h;
-        const R qw = R(1) / (R)heat_count(h, I.nz, I.nzh);
+        const size_t plane = (size_t)I.nx * I.ny;
+        // heat slices of the two calls (clamped to a valid slice when a call does not run)
+        const int h1 = heat_slice(c1 ? g - 1 : g, nz, I.nzh);
+        const int h0 = heat_slice(c0 ? g : g - 1, nz, I.nzh);
+        const size_t row1 = (size_t)h1 * plane + (size_t)y * I.nx;
+        const size_t row0 = (size_t)h0 * plane + (size_t)y * I.nx;
+        // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50,
+        // with DKint the running dk integral (src/headers/PhaseMatching.h:27-44)
+        const double zsc1 = I.dz / (double)g, zsc0 = I.dz / ((double)g + 1.0);
+        const double o1 = (double)(g - 1) + I.off[1][0], o0 = (double)g + I.off[0][0];
+        const double pol1 = c1 && I.poling ? I.poling[g - 1] : 0.0;
+        const double pol0 = c0 && I.poling ? I.poling[g] : 0.0;
+        const bool first = g - 1 == 0 || (c1 && heat_slice(g - 2, nz, I.nzh) != h1);
+        const R qw = c1 ? R(1) / (R)heat_count(h1, nz, I.nzh) : R(0);
         const R kp0 = (R)I.kp0, ks0 = (R)I.ks0, bf0 = (R)I.bf0, hce = (R)I.hce;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int x = tp + q * TPR;
-            const float npf = vox.n[q].x, nsf = vox.n[q].y;  // n_p(T), n_s(T) of this voxel
-            const R np = npf, ns = nsf;
-            const R kp = kp0 * recip<R>(npf), ks = ks0 * recip<R>(nsf), bfac = bf0 * ns;
-            // running dk integral (src/headers/PhaseMatching.h:27-44) and the
-            // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50;
-            // the three offsets are 0.25 dz apart in both schemes, so
-            // e(off + 1/4) = e(off) exp(i DKint dz / (4 (s + 1))).
-            const double dki = vox.dkb[q] + (double)step * vox.dk[q];
-            const C e0 = expi<C>(dki * o0 * zsc + pol);
-            const C ed = expi<C>(0.25 * dki * zsc);
-            const C e1 = cmul(e0, ed);
-            const C e[3] = {e0, e1, cmul(e1, ed)};
-            rk4_point(p[q], s[q], e, kp, ks, bfac, k);
-            if (deposit_q) {
+            const size_t v = (c1 ? row1 : row0) + x;
+            float2 n = I.nn[v];  // n_p(T), n_s(T) of the voxel
+            double dkb = I.dkbase[v];
+            double dkv = I.dk[v];
+            R np = n.x, ns = n.y;
+            R kp = kp0 * recip<R>(n.x), ks = ks0 * recip<R>(n.y), bfac = bf0 * ns;
+            C e0 = C{};
+            double ph = 0.0;
+            if (c1) {
+                const double dki = dkb + (double)(g - 1) * dkv;
+                ph = dki * o1 * zsc1 + pol1;
+                // the three offsets are 0.25 dz apart in both schemes:
+                // e(off + 1/4) = e(off) exp(i DKint dz / (4 (s + 1)))
+                e0 = expi<C>(ph);
+                const C ed = expi_small<C>(0.25 * dki * zsc1);
+                const C e1 = cmul(e0, ed);
+                const C e[3] = {e0, e1, cmul(e1, ed)};
+                rk4_point(p[q], s[q], e, kp, ks, bfac, k);
                 // Q = a_p I_p + a_s I_s + b_s I_s^2 -- src/headers/Tfield.h:260-276
                 const R Ip = hce * np * abs2(p[q]);
                 const R Is = hce * ns * abs2(s[q]);
                 const R qv = ((R)I.alpha_p * Ip + (R)I.alpha_s * Is + (R)I.beta_s * Is * Is) * qw;
-                Qr[x] = first ? (float)qv : (float)((R)Qr[x] + qv);
+                float* Qp = I.Q + row1 + x;
+                *Qp = first ? (float)qv : (float)((R)*Qp + qv);
+                emit(q);
+            }
+            if (c0) {
+                if (c1 && h0 != h1) {  // the call falls in the next heat slab
+                    n = I.nn[row0 + x];
+                    dkb = I.dkbase[row0 + x];
+                    dkv = I.dk[row0 + x];
+                    np = n.x;
+                    ns = n.y;
+                    kp = kp0 * recip<R>(n.x);
+                    ks = ks0 * recip<R>(n.y);
+                    bfac = bf0 * ns;
+                }
+                const double dki = dkb + (double)g * dkv;
+                const double ph0 = dki * o0 * zsc0 + pol0;
+                e0 = c1 ? cmul(e0, expi_small<C>(ph0 - ph)) : expi<C>(ph0);
+                const C ed = expi_small<C>(0.25 * dki * zsc0);
+                const C e1 = cmul(e0, ed);
+                const C e[3] = {e0, e1, cmul(e1, ed)};
+                rk4_point(p[q], s[q], e, kp, ks, bfac, k);
             }
         }
     }
@@ -197,19 +244,37 @@ __device__ __forceinline__ void nl_row(C (&p)[8], C (&s)[8], const MixedInst& I,
 //   0 < g < nz   : row IFFT -> R_end(g-1) [-> save state if g-1 == nz-2] -> R_start(g) -> row FFT
 //   g == nz      : row IFFT -> R_end(nz-1) -> spatial fields stored
 // grid (ceil(max_ny/ROWS), 1, n_instances), block (NX/8, ROWS)
+// Both fields of a row are transformed together (shared barriers, two LDS
+// buffers per row) in FP32; in FP64 one after the other through one buffer,
+// which halves the LDS footprint (occupancy is LDS-bound there).
+template <typename C> struct RowLds { static constexpr int BUFS = sizeof(C) == 16 ? 1 : 2; };
+
+template <typename C, int NX, bool INV>
+__device__ __forceinline__ void fft_pair(C (&f)[2][8], C* my, int ld, const double2* __restrict__ tw, int tp) {
+    if constexpr (RowLds<C>::BUFS == 1) {
+        fft_dist<C, NX, INV>(f[0], my, tw, tp);
+        fft_dist<C, NX, INV>(f[1], my, tw, tp);
+    } else {
+        fft_multi<C, NX, INV, 2>(f, my, ld, tw, tp);
+    }
+}
+
+// FP64 arithmetic: ask for 3 waves per SIMD (<= 168 VGPRs)
+template <typename C> struct RowWaves { static constexpr int W = sizeof(C) == 16 ? 3 : 1; };
+
 template <int NX, bool THERMAL, int ROWS, typename C>
-__global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __restrict__ insts, int g,
-                                                           const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NX / 8 * ROWS) __attribute__((amdgpu_waves_per_eu(RowWaves<C>::W)))
+void k_rowpass(const MixedInst* __restrict__ insts, int g, const double2* __restrict__ tw) {
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
-    __shared__ C lds[2 * ROWS * LD];
+    __shared__ C lds[RowLds<C>::BUFS * ROWS * LD];
     const MixedInst& I = insts[blockIdx.z];
     const int nz = I.nz;
     if (g > nz) return;
     const int y = blockIdx.x * ROWS + threadIdx.y;
     if (blockIdx.x * ROWS >= I.ny) return;
     const int tp = threadIdx.x;
-    C* my = lds + threadIdx.y * LD;  // pump buffer; SH buffer ROWS*LD further
+    C* my = lds + threadIdx.y * LD;  // pump buffer; with two buffers the SH one is ROWS*LD further
     const size_t base = (size_t)y * NX + tp;
     C f[2][8];  // f[0] pump, f[1] SH: transformed together (shared barriers)
     C(&p)[8] = f[0];
@@ -219,10 +284,6 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
         p[q] = from_store<C>(I.P[base + q * TPR]);
         s[q] = from_store<C>(I.S[base + q * TPR]);
     }
-    // heat-voxel values of the slab in registers, loaded once when both R
-    // calls of this launch fall in the same heat slice
-    VoxRow vox;
-    int hv = -1;
     // per-slice trace: the state before the SSFM step of slice g
     const bool trace = I.trP && g <= nz - 1 && g % I.trace_every == 0;
     if (trace && g == 0) {
@@ -233,40 +294,9 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_rowpass(const MixedInst* __re
             I.trS[i] = to_store(s[q], i, store_key(3, 1, g));
         }
     }
-    if (g > 0) {
-        fft_multi<C, NX, true, 2>(f, my, ROWS * LD, tw, tp);
-        if constexpr (THERMAL) {
-            hv = heat_slice(g - 1, nz, I.nzh);
-            load_vox<NX>(vox, I, hv, y, tp);
-        }
-        nl_row<NX, THERMAL, C>(p, s, I, g - 1, 1, y, tp, vox, hv, THERMAL);
-        if (g - 1 == nz - 2) {
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const size_t i = base + q * TPR;
-                I.outP[i] = to_store(p[q], i, store_key(2, 0, g));
-                I.outS[i] = to_store(s[q], i, store_key(2, 1, g));
-            }
-        }
-        if (trace) {
-            // same key as the saved output: a traced slice nz-1 is that state, bit for bit
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const size_t i = base + q * TPR;
-                I.trP[i] = to_store(p[q], i, store_key(2, 0, g));
-                I.trS[i] = to_store(s[q], i, store_key(2, 1, g));
-            }
-        }
-    }
-    if (g < nz) {
-        int h = -1;
-        if constexpr (THERMAL) {
-            h = heat_slice(g, nz, I.nzh);
-            if (h != hv) load_vox<NX>(vox, I, h, y, tp);
-        }
-        nl_row<NX, THERMAL, C>(p, s, I, g, 0, y, tp, vox, h, false);
-        fft_multi<C, NX, false, 2>(f, my, ROWS * LD, tw, tp);
-    }
+    if (g > 0) fft_pair<C, NX, true>(f, my, ROWS * LD, tw, tp);
+    nl_pair<NX, THERMAL, C>(p, s, I, g, y, tp, base, trace);
+    if (g < nz) fft_pair<C, NX, false>(f, my, ROWS * LD, tw, tp);
     const uint32_t kp = store_key(0, 0, g), ks = store_key(0, 1, g);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
