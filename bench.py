@@ -95,7 +95,8 @@ def run_once(h, ps):
 
 
 def committed_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    """Measured HBM bytes / algorithmic bytes of `kernel` from the committed
+    rocprofv3 counter summary (profiles/pmc_traffic.json)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
@@ -138,8 +139,11 @@ def kernel_roofline(h, ps):
             total += r.steps_done * p.nx * p.ny * BYTES_PER_PT["colpass"]
     ach = total / (ms * 1e-3) / 1e9
     tr = committed_traffic(name)
+    alg = total / n
+    traffic = round(alg * tr["traffic_over_algorithmic"], 1) if tr else None
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr["bytes_per_launch"] if tr else None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": "profiles/pmc_traffic.json (rocprofv3 TCC request counts + WRITE_SIZE)" if tr else None,
             "algorithmic_bytes_per_launch": round(total / n, 1), "avg_launch_us": round(ms / n * 1e3, 3),
             "launches": n, "timer": "hipEvent on the library stream"}, kt
 
