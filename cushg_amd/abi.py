@@ -37,6 +37,8 @@ PREC_FP64_PLAIN = 1
 PREC_FP32_STORAGE = 2
 HEAT_JACOBI_REFERENCE = 0
 HEAT_DIRECT = 1
+HEAT_BC_OPEN_TOP = 0
+HEAT_BC_OVEN_SURROUNDED = 1
 
 FFT_SIZES = (64, 128, 192, 256, 384, 512)
 
@@ -54,7 +56,7 @@ class ShgParams(C.Structure):
         ("power_W", C.c_double), ("waist_um", C.c_double), ("focal_um", C.c_double),
         ("T_C", C.c_double), ("T_inf_C", C.c_double), ("T_peltier_C", C.c_double),
         ("thermal", C.c_int32), ("precision", C.c_int32), ("heat_solver", C.c_int32),
-        ("max_outer", C.c_int32), ("check_every", C.c_int32), ("pad0", C.c_int32),
+        ("max_outer", C.c_int32), ("check_every", C.c_int32), ("heat_bc", C.c_int32),
         ("max_sweeps", C.c_int64),
         ("jacobi_tol", C.c_double), ("heat_tol_K", C.c_double),
         ("poling_phase", C.POINTER(C.c_double)),
@@ -78,7 +80,7 @@ class ShgParams(C.Structure):
         self.poling_phase = C.cast(buf, C.POINTER(C.c_double))
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("poling_phase", "pad0")}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "poling_phase"}
 
 
 class ShgResult(C.Structure):

@@ -140,6 +140,10 @@ int shg_validate(const shg_params* p) {
         return SHG_EINVAL;
     }
     if (p->thermal) {
+        if (p->heat_bc != SHG_HEAT_BC_OPEN_TOP && p->heat_bc != SHG_HEAT_BC_OVEN_SURROUNDED) {
+            set_error("unknown heat boundary set %d", p->heat_bc);
+            return SHG_EINVAL;
+        }
         if (p->heat_solver != SHG_HEAT_JACOBI_REFERENCE && p->heat_solver != SHG_HEAT_DIRECT) {
             set_error("unknown heat solver %d", p->heat_solver);
             return SHG_EINVAL;

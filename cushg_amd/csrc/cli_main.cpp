@@ -9,7 +9,7 @@
 // src/headers/Solver.h:32); here they are options:
 //   --nx= --ny= --nz= --nz-heat= --lx= --ly= --lz= (um)
 //   --crystal=spplt|ppln  --thermal=0|1  --precision=fp32|fp64
-//   --heat=jacobi|direct  --scheme=reference|strang  --max-outer=
+//   --heat=jacobi|direct  --heat-bc=open|oven  --scheme=reference|strang  --max-outer=
 //   --deff= (um/V) --period= (um) --alpha-p= --alpha-s= (1/um) --beta-s= (um/W)
 //   --rho-p= --rho-s= (rad) --out=DIR (default: current directory)
 #include <cstdio>
@@ -66,6 +66,8 @@ int main(int argc, char** argv) {
             p.precision = std::strcmp(v, "fp64") == 0 ? SHG_PREC_FP64_PLAIN : SHG_PREC_FP32_MIXED;
         } else if (opt(a, "--heat", &v)) {
             p.heat_solver = std::strcmp(v, "direct") == 0 ? SHG_HEAT_DIRECT : SHG_HEAT_JACOBI_REFERENCE;
+        } else if (opt(a, "--heat-bc", &v)) {
+            p.heat_bc = std::strcmp(v, "oven") == 0 ? SHG_HEAT_BC_OVEN_SURROUNDED : SHG_HEAT_BC_OPEN_TOP;
         } else if (opt(a, "--scheme", &v)) {
             p.scheme = std::strcmp(v, "strang") == 0 ? SHG_SCHEME_STRANG : SHG_SCHEME_REFERENCE;
         } else if (opt(a, "--max-outer", &v)) {

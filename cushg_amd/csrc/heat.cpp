@@ -29,6 +29,10 @@ HeatCoef heat_coef(const shg_params* p, int nzh) {
     c.nx = p->nx;
     c.ny = p->ny;
     c.nz = nzh;
+    c.oven = p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED;
+    c.x0 = c.oven ? 1 : 0;
+    c.nxu = c.oven ? p->nx - 2 : p->nx;
+    c.nyu = c.oven ? p->ny - 2 : p->ny - 1;
     const double dx = p->lx_um / (p->nx - 1), dy = p->ly_um / (p->ny - 1), dz = p->lz_um / (nzh - 1);
     const double kt = p->k_th, hc = p->h_conv;
     c.kth = kt;
@@ -135,17 +139,20 @@ struct AxisHost {
     int n = 0;
     std::vector<double> k, w, lam, q;
 };
-static bool axis_build(int n, double h, bool robin_lo, bool robin_hi, bool dirichlet_lo, double hk, AxisHost& a) {
+// An end marked Dirichlet couples its node to a fixed value one spacing
+// beyond it (a removed unknown, full-width control volume).
+static bool axis_build(int n, double h, bool robin_lo, bool robin_hi, bool dirichlet_lo, bool dirichlet_hi, double hk,
+                       AxisHost& a) {
     a.n = n;
     a.k.resize(n);
     a.w.resize(n);
     std::vector<double> diag(n), off(std::max(n - 1, 1), 0.0);
     for (int i = 0; i < n; i++) {
-        const bool lo_end = i == 0 && !dirichlet_lo, hi_end = i == n - 1;
+        const bool lo_end = i == 0 && !dirichlet_lo, hi_end = i == n - 1 && !dirichlet_hi;
         a.w[i] = (lo_end || hi_end) ? 0.5 * h : h;
         double kd = 0;
         if (i > 0 || dirichlet_lo) kd += 1.0 / h;
-        if (i < n - 1) kd += 1.0 / h;
+        if (i < n - 1 || dirichlet_hi) kd += 1.0 / h;
         if (i == 0 && robin_lo) kd += hk;
         if (i == n - 1 && robin_hi) kd += hk;
         a.k[i] = kd;
@@ -177,8 +184,8 @@ struct HeatGeom {
 
 static std::string geom_key(const shg_params* p, const HeatState& hs) {
     char b[256];
-    snprintf(b, sizeof b, "%d/%d/%d/%.17g/%.17g/%.17g/%.17g", hs.nx, hs.ny, hs.nzh, hs.dx, hs.dy, hs.dz,
-             p->h_conv / p->k_th);
+    snprintf(b, sizeof b, "%d/%d/%d/%.17g/%.17g/%.17g/%.17g/%d", hs.nx, hs.ny, hs.nzh, hs.dx, hs.dy, hs.dz,
+             p->h_conv / p->k_th, p->heat_bc);
     return b;
 }
 
@@ -186,12 +193,16 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
     const std::string key = geom_key(p, hs);
     auto it = h->heat_cache.find(key);
     if (it != h->heat_cache.end()) return it->second;
-    // axes: x (Robin both ends), y' = y-1 (Dirichlet below, Robin top), z (Robin both)
+    // open top: x (Robin both ends), y' = y-1 (Dirichlet below, Robin top), z (Robin both);
+    // oven-surrounded: x' = x-1 and y' = y-1 (Dirichlet both ends), z (Robin both)
     const double hk = p->h_conv / p->k_th;
+    const bool oven = p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED;
     AxisHost axes[3];
-    const bool ok = axis_build(hs.nx, hs.dx, true, true, false, hk, axes[0]) &&
-                    axis_build(hs.ny - 1, hs.dy, false, true, true, hk, axes[1]) &&
-                    axis_build(hs.nzh, hs.dz, true, true, false, hk, axes[2]);
+    const bool ok = (oven ? axis_build(hs.nx - 2, hs.dx, false, false, true, true, hk, axes[0])
+                          : axis_build(hs.nx, hs.dx, true, true, false, false, hk, axes[0])) &&
+                    (oven ? axis_build(hs.ny - 2, hs.dy, false, false, true, true, hk, axes[1])
+                          : axis_build(hs.ny - 1, hs.dy, false, true, true, false, hk, axes[1])) &&
+                    axis_build(hs.nzh, hs.dz, true, true, false, false, hk, axes[2]);
     if (!ok) {
         set_error("heat: tridiagonal eigen-solver did not converge");
         return nullptr;
@@ -252,7 +263,8 @@ static size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 size_t heat_bytes(const shg_params* p) {
     const size_t nzh = p->nz_heat > 0 ? p->nz_heat : p->nz;
     const size_t n3 = (size_t)p->nx * p->ny * nzh;
-    const size_t nu = (size_t)p->nx * (p->ny - 1) * nzh;
+    const HeatCoef c = heat_coef(p, (int)nzh);
+    const size_t nu = (size_t)c.nxu * c.nyu * nzh;
     const size_t rs = p->precision == SHG_PREC_FP64_PLAIN ? 8 : 4;
     size_t b = 4 * al256(8 * n3) + al256(8 * n3) + al256(4 * n3);  // T, Tn, dk, dkbase; nn, Q
     b += 2 * al256(rs * nu);                           // u, v
@@ -281,7 +293,8 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     hs.dy = p->ly_um / (p->ny - 1);
     hs.dz = p->lz_um / (hs.nzh - 1);
     const size_t n3 = (size_t)hs.nx * hs.ny * hs.nzh;
-    const size_t nu = (size_t)hs.nx * (hs.ny - 1) * hs.nzh;
+    const HeatCoef c = heat_coef(p, hs.nzh);
+    const size_t nu = (size_t)c.nxu * c.nyu * hs.nzh;
     const bool f64 = p->precision == SHG_PREC_FP64_PLAIN;
     Arena& A = h->arena;
     hs.T = A.carve<double>(n3);
@@ -290,7 +303,6 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     hs.dkbase = A.carve<double>(n3);
     hs.nn = A.carve<float2>(n3);
     hs.Q = A.carve<float>(n3);
-    const HeatCoef c = heat_coef(p, hs.nzh);
     hs.nblocks = heat_grid_blocks(c);
     hs.part = A.carve<double>(hs.nblocks);
     hs.red = A.carve<double>(8);
@@ -331,7 +343,8 @@ void heat_release(HeatState& hs) {
 
 int heat_init(shg_handle* h, const shg_params* p, HeatState& hs) {
     ProfScope ps(h, "heat_init");
-    SHG_HIP(launch_heat_init(h->stream, hs.T, hs.Q, hs.nx, hs.ny, hs.nzh, p->T_C, p->T_peltier_C));
+    SHG_HIP(launch_heat_init(h->stream, hs.T, hs.Q, hs.nx, hs.ny, hs.nzh, p->T_C, p->T_peltier_C,
+                             p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED));
     return SHG_OK;
 }
 
@@ -392,7 +405,7 @@ static int resid_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTab
 // correction pass: T += W^-1/2 S^-1 u (u from the residual pass); returns max|update|
 template <typename R>
 static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* max_corr) {
-    const int nx = hs.nx, nyu = hs.ny - 1, nz = hs.nzh;
+    const int nx = c.nxu, nyu = c.nyu, nz = hs.nzh;
     R* u = static_cast<R*>(dt.u);
     R* v = static_cast<R*>(dt.v);
     const R* qx = static_cast<const R*>(dt.q[0]);

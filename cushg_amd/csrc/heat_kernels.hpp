@@ -5,9 +5,13 @@
 
 namespace shg {
 
-// Constants of the open-top Jacobi sweep (src/headers/Tfield.h:99-106)
+// Constants of the Jacobi sweep (src/headers/Tfield.h:99-106) and the
+// unknown block of the steady-state problem: x in [x0, x0 + nxu),
+// y in [1, 1 + nyu), every z (the other nodes are Dirichlet oven nodes).
 struct HeatCoef {
     int nx, ny, nz;
+    int oven;      // SHG_HEAT_BC_OVEN_SURROUNDED
+    int x0, nxu, nyu;
     double kth, Tinf;
     double idx2, idy2, idz2, L2, ikth;
     double Bix, Biy, Biz;
@@ -20,7 +24,7 @@ struct HeatCoef {
 // 1D factors of the tensor-product finite-volume form
 //   A = Kx (x) Wy (x) Wz + Wx (x) Ky (x) Wz + Wx (x) Wy (x) Kz
 // (kx/ky/kz: diagonals of K; wx/wy/wz: control-volume widths; lx/ly/lz:
-// eigenvalues of W^-1/2 K W^-1/2).  y runs over the unknowns y = 1..ny-1.
+// eigenvalues of W^-1/2 K W^-1/2), over the unknown block of HeatCoef.
 struct HeatAxes {
     const double *kx, *ky, *kz;
     const double *wx, *wy, *wz;
@@ -36,12 +40,13 @@ template <typename R>
 hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* T, const float* Q,
                              R* u, double* part_sq);
 template <typename R>
-hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nx, int nyu, int nz);
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz);
 template <typename R>
 hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
                             double* part_max);
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out);
-hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel);
+hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel,
+                            bool oven);
 hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, long n, int crystal, double lp,
                           double Lambda, double ea, double eb);
 hipError_t launch_heat_dkbase(hipStream_t st, const double* dk, double* base, int n2, int nzh, int nz);

@@ -15,10 +15,19 @@ struct Nb {
     double c, xm, xp, ym, yp, zm, zp;
 };
 
-// One application of the Jacobi map of src/headers/Tfield.h:93-201 at node
-// (x, y, z), y >= 1 (the y = 0 plane is the Dirichlet oven), given the
-// values of its neighbours (those outside the crystal are not used).
+// One application of the Jacobi map at a non-Dirichlet node (x, y, z),
+// given the values of its neighbours (those outside the crystal are not
+// used).  Open top: src/headers/Tfield.h:93-201 (y = 0 is the oven plane).
+// Oven-surrounded: src/headers/Tfield.h:205-240 (the four lateral planes
+// are the oven; inner nodes, and convective z faces without Q).
 __device__ __forceinline__ double jacobi_vals(const HeatCoef& c, int x, int y, int z, const Nb& t, float q) {
+    if (c.oven) {
+        if (z > 0 && z < c.nz - 1)  // inner points, :217-219
+            return c.L2 * ((t.xp + t.xm) * c.idx2 + (t.yp + t.ym) * c.idy2 + (t.zp + t.zm) * c.idz2 +
+                           (double)q * c.ikth);
+        const double zn = z == 0 ? t.zp : t.zm;  // back/front faces, :224-232
+        return (zn + c.az * (t.xp + t.xm) + c.bz * (t.yp + t.ym) + c.Biz * c.Tinf) * c.Fz;
+    }
     const bool xi = x > 0 && x < c.nx - 1, yi = y < c.ny - 1, zi = z > 0 && z < c.nz - 1;
     const bool xe = !xi, ze = !zi;
     const double xn = x == 0 ? t.xp : t.xm;  // inward x neighbour
@@ -42,7 +51,8 @@ __device__ __forceinline__ double jacobi_vals(const HeatCoef& c, int x, int y, i
     return (c.AV * xn + c.BV * t.ym + c.CV * zn + Tinf * c.BiD) * c.GV;
 }
 
-// Walks the z column of node (x, y), y >= 1, calling f(z, index, F(T), F(T) - T).
+// Walks the z column of a non-Dirichlet node (x, y), calling
+// f(z, index, F(T), F(T) - T).
 template <typename Fn>
 __device__ __forceinline__ void jacobi_column(const HeatCoef& c, const double* __restrict__ T,
                                               const float* __restrict__ Q, int x, int y, Fn&& f) {
@@ -96,9 +106,10 @@ __global__ __launch_bounds__(256) void k_heat_sweep(HeatCoef c, const double* __
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     double d2 = 0.0;
     if (x < c.nx && y < c.ny) {
-        if (y == 0) {
+        const bool fixed = y == 0 || (c.oven && (x == 0 || x == c.nx - 1 || y == c.ny - 1));
+        if (fixed) {  // oven nodes are copied through
             for (int z = 0; z < c.nz; z++) {
-                const size_t i = ((size_t)z * c.ny) * c.nx + x;
+                const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
                 Tf[i] = Ti[i];
             }
         } else {
@@ -116,7 +127,7 @@ __global__ __launch_bounds__(256) void k_heat_sweep(HeatCoef c, const double* __
 
 // FP64 residual of the steady state: delta = F(T) - T (the reference's
 // one-sweep change) and the finite-volume residual r = A_ii * delta, written
-// scaled by W^-1/2 into the unknown layout u[z][y-1][x] that the
+// scaled by W^-1/2 into the unknown layout u[z][y-1][x-x0] that the
 // fast-diagonalisation transforms consume; partial sums of delta^2.
 template <typename R>
 __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, const double* __restrict__ T,
@@ -125,15 +136,15 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
     __shared__ double red[4];
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     double d2 = 0.0;
-    if (x < c.nx && y >= 1 && y < c.ny) {
-        const int j = y - 1;
-        const double wx = a.wx[x], wy = a.wy[j], kx = a.kx[x], ky = a.ky[j];
-        const size_t nyu = (size_t)(c.ny - 1);
+    const int xu = x - c.x0, j = y - 1;
+    if (xu >= 0 && xu < c.nxu && j >= 0 && j < c.nyu) {
+        const double wx = a.wx[xu], wy = a.wy[j], kx = a.kx[xu], ky = a.ky[j];
+        const size_t nyu = (size_t)c.nyu, nxu = (size_t)c.nxu;
         jacobi_column(c, T, Q, x, y, [&](int z, size_t, double, double d) {
             d2 += d * d;
             const double wz = a.wz[z];
             const double aii = kx * wy * wz + wx * ky * wz + wx * wy * a.kz[z];
-            u[((size_t)z * nyu + j) * c.nx + x] = (R)(aii * d * rsqrt(wx * wy * wz));
+            u[((size_t)z * nyu + j) * nxu + xu] = (R)(aii * d * rsqrt(wx * wy * wz));
         });
     }
     const double s = block_reduce(d2, false, red);
@@ -143,11 +154,11 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
 // spectral division by lambda_x + lambda_y + lambda_z
 template <typename R>
 __global__ void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx, const double* __restrict__ ly,
-                            const double* __restrict__ lz, int nx, int nyu, int nz) {
-    const long n = (long)nx * nyu * nz;
+                            const double* __restrict__ lz, int nxu, int nyu, int nz) {
+    const long n = (long)nxu * nyu * nz;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
-        const int x = (int)(i % nx);
-        const long r = i / nx;
+        const int x = (int)(i % nxu);
+        const long r = i / nxu;
         const int j = (int)(r % nyu);
         const int z = (int)(r / nyu);
         v[i] = (R)((double)v[i] / (lx[x] + ly[j] + lz[z]));
@@ -161,12 +172,12 @@ __global__ __launch_bounds__(256) void k_fd_update(HeatCoef c, HeatAxes a, const
     __shared__ double red[4];
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     double m = 0.0;
-    if (x < c.nx && y >= 1 && y < c.ny) {
-        const int j = y - 1;
-        const double wxy = a.wx[x] * a.wy[j];
-        const size_t nyu = (size_t)(c.ny - 1);
+    const int xu = x - c.x0, j = y - 1;
+    if (xu >= 0 && xu < c.nxu && j >= 0 && j < c.nyu) {
+        const double wxy = a.wx[xu] * a.wy[j];
+        const size_t nyu = (size_t)c.nyu, nxu = (size_t)c.nxu;
         for (int z = 0; z < c.nz; z++) {
-            const double corr = (double)u[((size_t)z * nyu + j) * c.nx + x] * rsqrt(wxy * a.wz[z]);
+            const double corr = (double)u[((size_t)z * nyu + j) * nxu + xu] * rsqrt(wxy * a.wz[z]);
             T[((size_t)z * c.ny + y) * c.nx + x] += corr;
             m = fmax(m, fabs(corr));
         }
@@ -190,14 +201,17 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part,
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// T = T_C everywhere, y = 0 plane = T_peltier (Tfield::setTemperature /
-// setBottomOvens, src/headers/Tfield.h:28-61); Q = 0 (setInitialQ)
+// T = T_C everywhere, the oven nodes = T_peltier (Tfield::setTemperature /
+// setBottomOvens, src/headers/Tfield.h:28-61; setOvenSurrounded, :63-90,
+// :333-348); Q = 0 (setInitialQ)
 __global__ void k_heat_init(double* __restrict__ T, float* __restrict__ Q, int nx, int ny, int nz, double Tc,
-                            double Tpel) {
+                            double Tpel, int oven) {
     const long n = (long)nx * ny * nz;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
+        const int x = (int)(i % nx);
         const int y = (int)((i / nx) % ny);
-        T[i] = y == 0 ? Tpel : Tc;
+        const bool fixed = y == 0 || (oven && (x == 0 || x == nx - 1 || y == ny - 1));
+        T[i] = fixed ? Tpel : Tc;
         Q[i] = 0.0f;
     }
 }
@@ -275,11 +289,11 @@ template hipError_t launch_heat_resid<double>(hipStream_t, const HeatCoef&, cons
                                               const float*, double*, double*);
 
 template <typename R>
-hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nx, int nyu, int nz) {
-    const long n = (long)nx * nyu * nz;
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz) {
+    const long n = (long)nxu * nyu * nz;
     long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL((k_fd_divide<R>), dim3((unsigned)nb), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nx, nyu, nz);
+    hipLaunchKernelGGL((k_fd_divide<R>), dim3((unsigned)nb), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu, nyu, nz);
     return hipGetLastError();
 }
 template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int);
@@ -301,8 +315,9 @@ hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max,
     return hipGetLastError();
 }
 
-hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel) {
-    hipLaunchKernelGGL(k_heat_init, dim3(2048), dim3(256), 0, st, T, Q, nx, ny, nz, Tc, Tpel);
+hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel,
+                            bool oven) {
+    hipLaunchKernelGGL(k_heat_init, dim3(2048), dim3(256), 0, st, T, Q, nx, ny, nz, Tc, Tpel, oven ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -83,6 +83,15 @@ enum { SHG_PREC_FP32_MIXED = 0, SHG_PREC_FP64_PLAIN = 1, SHG_PREC_FP32_STORAGE =
  * by fast diagonalisation, refined until the update is below heat_tol_K. */
 enum { SHG_HEAT_JACOBI_REFERENCE = 0, SHG_HEAT_DIRECT = 1 };
 
+/* Thermal boundary set.  OPEN_TOP is the reference's (kernelHeatEquationTopOpen,
+ * src/headers/Tfield.h:93-201): the y = 0 plane held at T_peltier, all other
+ * faces convective to T_inf.  OVEN_SURROUNDED is the reference's unused
+ * variant (kernelSetOvenSurrounded / kernelHeatEquationOvenSurrounded,
+ * src/headers/Tfield.h:64-90,205-240,333-348): the four lateral planes
+ * x = 0, x = lx, y = 0, y = ly held at T_peltier, the faces z = 0 and
+ * z = lz convective. */
+enum { SHG_HEAT_BC_OPEN_TOP = 0, SHG_HEAT_BC_OVEN_SURROUNDED = 1 };
+
 typedef struct shg_params {
     /* grid (src/cuSHG.cu:11-15, src/headers/Crystal.h:31-36) */
     int32_t nx, ny;          /* transverse points; even; supported sizes below */
@@ -119,7 +128,7 @@ typedef struct shg_params {
     int32_t heat_solver;     /* SHG_HEAT_* */
     int32_t max_outer;       /* outer thermo-optic passes; reference 20 */
     int32_t check_every;     /* Jacobi check period; reference 5000 */
-    int32_t pad0;
+    int32_t heat_bc;         /* SHG_HEAT_BC_*; reference: OPEN_TOP */
     int64_t max_sweeps;      /* Jacobi safety cap (SHG_ENOCONV beyond) */
     double jacobi_tol;       /* ||dT||_2 per sweep [K]; reference 5e-4 */
     double heat_tol_K;       /* DIRECT refinement tolerance [K] */

@@ -494,9 +494,12 @@ void orc_ssfm_step(const shg_params* p, int32_t s, double* Pd, double* Sd) {
 
 #define I3(x, y, z) (((size_t)(z) * g.ny + (y)) * g.nx + (x))
 
-/* One Jacobi sweep Tf = F(Ti) with the open-top boundary set:
- * src/headers/Tfield.h:93-201.  The y = 0 plane is the Dirichlet oven and
- * is copied through (the reference never writes it). */
+/* One Jacobi sweep Tf = F(Ti).  Open-top boundary set (the reference's):
+ * src/headers/Tfield.h:93-201, the y = 0 plane is the Dirichlet oven and
+ * is copied through (the reference never writes it).  Oven-surrounded set
+ * (SHG_HEAT_BC_OVEN_SURROUNDED): src/headers/Tfield.h:205-240, the four
+ * lateral planes are the Dirichlet oven, inner nodes as above, the z = 0 and
+ * z = lz faces convective without Q. */
 void orc_jacobi_sweep(const shg_params* p, int32_t nzh, const double* Ti, double* Tf,
                       const double* Q) {
     grid_t g;
@@ -525,6 +528,21 @@ void orc_jacobi_sweep(const shg_params* p, int32_t nzh, const double* Ti, double
         for (int y = 0; y < NY; y++)
             for (int x = 0; x < NX; x++) {
                 const size_t i = I3(x, y, z);
+                if (p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED) {
+                    if (x == 0 || x == NX - 1 || y == 0 || y == NY - 1) { Tf[i] = Ti[i]; continue; }
+                    if (z > 0 && z < NZ - 1) { /* inner points, :217-219 */
+                        Tf[i] = L2 * ((Ti[I3(x + 1, y, z)] + Ti[I3(x - 1, y, z)]) / (dx * dx) +
+                                      (Ti[I3(x, y + 1, z)] + Ti[I3(x, y - 1, z)]) / (dy * dy) +
+                                      (Ti[I3(x, y, z + 1)] + Ti[I3(x, y, z - 1)]) / (dz * dz) +
+                                      Q[i] / kt);
+                    } else { /* back/front faces, :224-232 */
+                        const int zn = z == 0 ? z + 1 : z - 1;
+                        Tf[i] = (Ti[I3(x, y, zn)] + az * (Ti[I3(x + 1, y, z)] + Ti[I3(x - 1, y, z)]) +
+                                 bz * (Ti[I3(x, y + 1, z)] + Ti[I3(x, y - 1, z)]) + Biz * Tinf) /
+                                (1 + 2 * az + 2 * bz + Biz);
+                    }
+                    continue;
+                }
                 if (y == 0) { Tf[i] = Ti[i]; continue; }
                 const int xi = x > 0 && x < NX - 1, yi = y < NY - 1, zi = z > 0 && z < NZ - 1;
                 const int xe = !xi, ze = !zi; /* on an x / z boundary */
@@ -640,10 +658,10 @@ typedef struct {
 } heat_axis;
 
 /* K = conductances (1/h between neighbours, + h_conv/k_th at Robin ends);
- * robin_lo/hi select the Robin ends; dirichlet_lo couples node 0 to a fixed
- * value one spacing below (removed unknown). */
+ * robin_lo/hi select the Robin ends; dirichlet_lo/hi couple the end node to
+ * a fixed value one spacing beyond it (a removed unknown). */
 static int heat_axis_init(heat_axis* a, int n, double h, int robin_lo, int robin_hi,
-                          int dirichlet_lo, double hk) {
+                          int dirichlet_lo, int dirichlet_hi, double hk) {
     a->n = n;
     a->w = (double*)malloc(sizeof(double) * n);
     a->lam = (double*)malloc(sizeof(double) * n);
@@ -652,11 +670,11 @@ static int heat_axis_init(heat_axis* a, int n, double h, int robin_lo, int robin
     double* dg = (double*)malloc(sizeof(double) * n);
     double* of = (double*)malloc(sizeof(double) * (n > 1 ? n - 1 : 1));
     for (int i = 0; i < n; i++) {
-        const int lo_end = i == 0 && !dirichlet_lo, hi_end = i == n - 1;
+        const int lo_end = i == 0 && !dirichlet_lo, hi_end = i == n - 1 && !dirichlet_hi;
         a->w[i] = (lo_end || hi_end) ? 0.5 * h : h;
         double kd = 0;
         if (i > 0 || dirichlet_lo) kd += 1.0 / h;
-        if (i < n - 1) kd += 1.0 / h;
+        if (i < n - 1 || dirichlet_hi) kd += 1.0 / h;
         if (i == 0 && robin_lo) kd += hk;
         if (i == n - 1 && robin_hi) kd += hk;
         dg[i] = kd / a->w[i];
@@ -726,53 +744,62 @@ int orc_heat_solve_direct(const shg_params* p, int32_t nzh, const double* Q, dou
     grid_t g;
     grid_init(p, &g);
     g.nz = nzh;
-    const int NX = g.nx, NY = g.ny, NZ = nzh, NYu = NY - 1;
+    const int NX = g.nx, NY = g.ny, NZ = nzh;
+    const int oven = p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED;
+    /* unknowns: x in [X0, X0+NXu), y in [1, 1+NYu), all z */
+    const int X0 = oven ? 1 : 0, NXu = oven ? NX - 2 : NX, NYu = oven ? NY - 2 : NY - 1;
     const double dx = g.dx, dy = g.dy, dz = p->lz_um / (nzh - 1);
     const double hk = p->h_conv / p->k_th, Tinf = p->T_inf_C, Tpel = p->T_peltier_C;
     heat_axis ax, ay, az;
-    int rc = heat_axis_init(&ax, NX, dx, 1, 1, 0, hk);
-    rc |= heat_axis_init(&ay, NYu, dy, 0, 1, 1, hk);
-    rc |= heat_axis_init(&az, NZ, dz, 1, 1, 0, hk);
-    const size_t NU = (size_t)NX * NYu * NZ;
+    int rc = oven ? heat_axis_init(&ax, NXu, dx, 0, 0, 1, 1, hk) : heat_axis_init(&ax, NX, dx, 1, 1, 0, 0, hk);
+    rc |= oven ? heat_axis_init(&ay, NYu, dy, 0, 0, 1, 1, hk) : heat_axis_init(&ay, NYu, dy, 0, 1, 1, 0, hk);
+    rc |= heat_axis_init(&az, NZ, dz, 1, 1, 0, 0, hk);
+    const size_t NU = (size_t)NXu * NYu * NZ;
     double* u = (double*)malloc(sizeof(double) * NU);
     double* v = (double*)malloc(sizeof(double) * NU);
     if (rc || !u || !v) { rc = -1; goto done; }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < NZ; z++)
         for (int j = 0; j < NYu; j++)
-            for (int x = 0; x < NX; x++) {
-                const int y = j + 1;
-                const double wx = ax.w[x], wy = ay.w[j], wz = az.w[z];
+            for (int xu = 0; xu < NXu; xu++) {
+                const int y = j + 1, x = xu + X0;
+                const double wx = ax.w[xu], wy = ay.w[j], wz = az.w[z];
                 double b = 0;
                 const int xi = x > 0 && x < NX - 1, yi = y < NY - 1, zi = z > 0 && z < NZ - 1;
                 if (xi && yi && zi) b += Q[I3(x, y, z)] / p->k_th * wx * wy * wz; /* Q8 */
-                if (x == 0) b += hk * Tinf * wy * wz;
-                if (x == NX - 1) b += hk * Tinf * wy * wz;
                 if (z == 0) b += hk * Tinf * wx * wy;
                 if (z == NZ - 1) b += hk * Tinf * wx * wy;
-                if (y == NY - 1) b += hk * Tinf * wx * wz;
                 if (j == 0) b += Tpel / dy * wx * wz;
-                u[((size_t)z * NYu + j) * NX + x] = b / sqrt(wx * wy * wz);
+                if (oven) {
+                    if (j == NYu - 1) b += Tpel / dy * wx * wz;
+                    if (xu == 0) b += Tpel / dx * wy * wz;
+                    if (xu == NXu - 1) b += Tpel / dx * wy * wz;
+                } else {
+                    if (x == 0) b += hk * Tinf * wy * wz;
+                    if (x == NX - 1) b += hk * Tinf * wy * wz;
+                    if (y == NY - 1) b += hk * Tinf * wx * wz;
+                }
+                u[((size_t)z * NYu + j) * NXu + xu] = b / sqrt(wx * wy * wz);
             }
-    apply_axis(ax.q, 0, NX, NYu, NZ, u, v);
-    apply_axis(ay.q, 1, NX, NYu, NZ, v, u);
-    apply_axis(az.q, 2, NX, NYu, NZ, u, v);
+    apply_axis(ax.q, 0, NXu, NYu, NZ, u, v);
+    apply_axis(ay.q, 1, NXu, NYu, NZ, v, u);
+    apply_axis(az.q, 2, NXu, NYu, NZ, u, v);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < NZ; z++)
         for (int j = 0; j < NYu; j++)
-            for (int x = 0; x < NX; x++)
-                v[((size_t)z * NYu + j) * NX + x] /= ax.lam[x] + ay.lam[j] + az.lam[z];
-    apply_axis(az.qt, 2, NX, NYu, NZ, v, u);
-    apply_axis(ay.qt, 1, NX, NYu, NZ, u, v);
-    apply_axis(ax.qt, 0, NX, NYu, NZ, v, u);
+            for (int xu = 0; xu < NXu; xu++)
+                v[((size_t)z * NYu + j) * NXu + xu] /= ax.lam[xu] + ay.lam[j] + az.lam[z];
+    apply_axis(az.qt, 2, NXu, NYu, NZ, v, u);
+    apply_axis(ay.qt, 1, NXu, NYu, NZ, u, v);
+    apply_axis(ax.qt, 0, NXu, NYu, NZ, v, u);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int z = 0; z < NZ; z++)
         for (int y = 0; y < NY; y++)
             for (int x = 0; x < NX; x++) {
-                if (y == 0) { T[I3(x, y, z)] = Tpel; continue; }
-                const int j = y - 1;
-                T[I3(x, y, z)] = u[((size_t)z * NYu + j) * NX + x] /
-                                 sqrt(ax.w[x] * ay.w[j] * az.w[z]);
+                const int j = y - 1, xu = x - X0;
+                if (j < 0 || j >= NYu || xu < 0 || xu >= NXu) { T[I3(x, y, z)] = Tpel; continue; }
+                T[I3(x, y, z)] = u[((size_t)z * NYu + j) * NXu + xu] /
+                                 sqrt(ax.w[xu] * ay.w[j] * az.w[z]);
             }
 done:
     free(u);
@@ -862,6 +889,11 @@ int orc_run(const shg_params* p, shg_result* r) {
         for (size_t i = 0; i < NV; i++) T[i] = p->T_C;             /* setTemperature */
         for (int z = 0; z < g.nzh; z++)                            /* setBottomOvens */
             for (int x = 0; x < g.nx; x++) T[I3(x, 0, z)] = p->T_peltier_C;
+        if (p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED)              /* setOvenSurrounded */
+            for (int z = 0; z < g.nzh; z++)
+                for (int y = 0; y < g.ny; y++)
+                    for (int x = 0; x < g.nx; x++)
+                        if (x == 0 || x == g.nx - 1 || y == g.ny - 1) T[I3(x, y, z)] = p->T_peltier_C;
         const int max_outer = p->max_outer > 0 ? p->max_outer : 20;
         const int chk = p->check_every > 0 ? p->check_every : 5000;
         for (int gi = 0; gi < max_outer; gi++) {

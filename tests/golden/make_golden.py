@@ -5,9 +5,9 @@ this path (its repository has no tests at all), so every fixture here comes
 from an independent source:
   fft_vectors.npz   numpy.fft (pocketfft) outputs for seeded complex inputs
                     of every GPU transform length, 1D and 2D.
-  heat_fv.npz       steady states of the open-top heat problem assembled
+  heat_fv.npz       steady states of the open-top and oven-surrounded heat problems assembled
                     independently as a finite-volume sparse matrix (face
-                    conductances k*A/d, convective faces h*A, Dirichlet bottom)
+                    conductances k*A/d, convective faces h*A, Dirichlet oven planes)
                     and solved with scipy.sparse.linalg.spsolve.
   sellmeier.json    n(lambda, T) and dk(T) evaluated with Python's decimal
                     module at 40 significant digits from the published
@@ -43,28 +43,39 @@ def fft_vectors():
     np.savez_compressed(os.path.join(HERE, "fft_vectors.npz"), **out)
 
 
-def fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Q):
+def fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Q, bc="open"):
     """Finite-volume steady heat equation on the node grid of the reference
     (dx = lx/(nx-1) ...), each node owning the cell [x-dx/2, x+dx/2] clipped
-    to the crystal; faces x=0, x=lx, y=ly, z=0, z=lz convective (h), the
-    y = 0 plane held at T_pel, Q applied to interior nodes only."""
+    to the crystal, Q applied to interior nodes only.
+      bc="open": faces x=0, x=lx, y=ly, z=0, z=lz convective (h), the y = 0
+                 plane held at T_pel (the reference's open-top oven);
+      bc="oven": the four lateral planes x=0, x=lx, y=0, y=ly held at T_pel,
+                 faces z=0, z=lz convective (the oven-surrounded variant)."""
     dx, dy, dz = lx / (nx - 1), ly / (ny - 1), lz / (nz - 1)
 
     def width(i, n, d):
         return d / 2 if i in (0, n - 1) else d
 
+    def fixed(x, y):
+        if bc == "oven":
+            return x in (0, nx - 1) or y in (0, ny - 1)
+        return y == 0
+
     idx = -np.ones((nz, ny, nx), dtype=np.int64)
     cnt = 0
     for z in range(nz):
-        for y in range(1, ny):
+        for y in range(ny):
             for x in range(nx):
-                idx[z, y, x] = cnt
-                cnt += 1
+                if not fixed(x, y):
+                    idx[z, y, x] = cnt
+                    cnt += 1
     rows, cols, vals = [], [], []
     b = np.zeros(cnt)
     for z in range(nz):
-        for y in range(1, ny):
+        for y in range(ny):
             for x in range(nx):
+                if fixed(x, y):
+                    continue
                 i = idx[z, y, x]
                 wx, wy, wz = width(x, nx, dx), width(y, ny, dy), width(z, nz, dz)
                 diag = 0.0
@@ -78,7 +89,7 @@ def fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Q):
                         continue
                     g = k * area / d
                     diag += g
-                    if yy == 0:
+                    if fixed(xx, yy):
                         b[i] += g * T_pel
                     else:
                         rows.append(i)
@@ -86,10 +97,11 @@ def fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Q):
                         vals.append(-g)
                 # convective faces
                 conv = 0.0
-                if x == 0 or x == nx - 1:
-                    conv += h * wy * wz
-                if y == ny - 1:
-                    conv += h * wx * wz
+                if bc == "open":
+                    if x == 0 or x == nx - 1:
+                        conv += h * wy * wz
+                    if y == ny - 1:
+                        conv += h * wx * wz
                 if z == 0 or z == nz - 1:
                     conv += h * wx * wy
                 diag += conv
@@ -104,9 +116,10 @@ def fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, T_inf, T_pel, Q):
     t = spla.spsolve(A.tocsc(), b)
     T = np.full((nz, ny, nx), T_pel)
     for z in range(nz):
-        for y in range(1, ny):
+        for y in range(ny):
             for x in range(nx):
-                T[z, y, x] = t[idx[z, y, x]]
+                if not fixed(x, y):
+                    T[z, y, x] = t[idx[z, y, x]]
     return T
 
 
@@ -125,6 +138,13 @@ def heat_fv():
         cases[f"spec{ci}"] = np.array([nx, ny, nz, lx, ly, lz, k, h, Ti, Tp], dtype=np.float64)
         cases[f"Q{ci}"] = Q
         cases[f"T{ci}"] = T
+    # the oven-surrounded variant on the same three geometries (own sources)
+    for ci, (nx, ny, nz, lx, ly, lz, k, h, Ti, Tp, qs) in enumerate(specs):
+        Q = rng.random((nz, ny, nx)) * qs
+        T = fv_heat_solution(nx, ny, nz, lx, ly, lz, k, h, Ti, Tp, Q, bc="oven")
+        cases[f"ospec{ci}"] = np.array([nx, ny, nz, lx, ly, lz, k, h, Ti, Tp], dtype=np.float64)
+        cases[f"oQ{ci}"] = Q
+        cases[f"oT{ci}"] = T
     np.savez_compressed(os.path.join(HERE, "heat_fv.npz"), **cases)
 
 

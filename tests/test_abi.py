@@ -114,6 +114,7 @@ def _bad(**kw):
     (dict(scheme=9), "scheme"),
     (dict(precision=3), "precision"),
     (dict(thermal=1, heat_solver=4), "heat"),
+    (dict(thermal=1, heat_bc=5), "boundary"),
     (dict(Lambda_um=0.0), "Lambda"),
 ])
 def test_validate_rejects(kw, msg):

@@ -112,3 +112,44 @@ def test_spplt256_thermal_scaled_vs_oracle(handle, form):
     assert rg.outer_iters == ro.outer_iters
     assert rel(rg.eta, ro.eta) <= tol
     assert rel(rg.peak_dT_K, ro.peak_dT_K) <= tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", FORMS)
+def test_oven_surrounded_direct_vs_oracle(handle, form):
+    """The oven-surrounded boundary set (four lateral planes at the oven
+    temperature, convective end faces; src/headers/Tfield.h:205-240)."""
+    p = small_thermal(nzh=12)
+    p.heat_bc = abi.HEAT_BC_OVEN_SURROUNDED
+    p.precision = form
+    print(f"case oven-surrounded direct precision={form}", flush=True)
+    ro, o = O.run(p, temperature=True)
+    rg, g = handle.run(p, temperature=True)
+    print(f"  eta gpu={rg.eta:.10e} cpu={ro.eta:.10e} rel={rel(rg.eta, ro.eta):.2e} "
+          f"dT gpu={rg.peak_dT_K:.8f} cpu={ro.peak_dT_K:.8f} outer {rg.outer_iters}/{ro.outer_iters}")
+    assert ro.peak_dT_K > 0.05
+    assert rg.outer_iters == ro.outer_iters
+    assert rel(rg.eta, ro.eta) <= 1e-5
+    assert rel(rg.peak_dT_K, ro.peak_dT_K) <= 1e-5
+    assert np.abs(g["T"] - o["T"]).max() <= 1e-5 * ro.peak_dT_K
+    # the lateral planes stay at the oven temperature
+    for sl in (g["T"][:, :, 0], g["T"][:, :, -1], g["T"][:, 0, :], g["T"][:, -1, :]):
+        assert np.all(sl == p.T_peltier_C)
+
+
+@pytest.mark.gpu
+def test_oven_surrounded_jacobi_reference_loop_vs_oracle(handle):
+    p = small_thermal(n=64, nz=16, nzh=6, solver=abi.HEAT_JACOBI_REFERENCE)
+    p.heat_bc = abi.HEAT_BC_OVEN_SURROUNDED
+    p.lz_um = 2000.0
+    p.focal_um = 1000.0
+    p.check_every = 200
+    p.jacobi_tol = 1e-3
+    p.max_outer = 4
+    ro, o = O.run(p, temperature=True)
+    rg, g = handle.run(p, temperature=True)
+    print(f"  sweeps gpu={rg.jacobi_sweeps} cpu={ro.jacobi_sweeps} eta rel={rel(rg.eta, ro.eta):.2e}")
+    assert rg.jacobi_sweeps == ro.jacobi_sweeps
+    assert rg.outer_iters == ro.outer_iters
+    assert np.abs(g["T"] - o["T"]).max() <= 1e-5 * max(ro.peak_dT_K, 1e-3)
+    assert rel(rg.eta, ro.eta) <= 1e-5

@@ -233,44 +233,67 @@ def heatgold():
     return np.load(os.path.join(GOLD, "heat_fv.npz"))
 
 
-def _heat_params(spec):
+def _heat_params(spec, bc=abi.HEAT_BC_OPEN_TOP):
     nx, ny, nz, lx, ly, lz, k, h, Ti, Tp = spec
     p = abi.default_params()
     p.nx, p.ny = int(nx), int(ny)
     p.nz = int(nz)
     p.lx_um, p.ly_um, p.lz_um = lx, ly, lz
     p.k_th, p.h_conv, p.T_inf_C, p.T_peltier_C = k, h, Ti, Tp
+    p.heat_bc = bc
     return p, int(nz)
 
 
+# the open-top set (the reference's) and the oven-surrounded set, both
+# against their own independently assembled scipy solutions
+BCS = [pytest.param("", abi.HEAT_BC_OPEN_TOP, id="open-top"),
+       pytest.param("o", abi.HEAT_BC_OVEN_SURROUNDED, id="oven-surrounded")]
+
+
+@pytest.mark.parametrize("pre,bc", BCS)
 @pytest.mark.parametrize("case", [0, 1, 2])
-def test_direct_heat_solve_vs_scipy_fv(heatgold, case):
-    p, nzh = _heat_params(heatgold[f"spec{case}"])
-    T = O.heat_solve_direct(p, nzh, heatgold[f"Q{case}"])
-    ref = heatgold[f"T{case}"]
+def test_direct_heat_solve_vs_scipy_fv(heatgold, case, pre, bc):
+    p, nzh = _heat_params(heatgold[f"{pre}spec{case}"], bc)
+    T = O.heat_solve_direct(p, nzh, heatgold[f"{pre}Q{case}"])
+    ref = heatgold[f"{pre}T{case}"]
     assert np.abs(T - ref).max() < 1e-9 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("pre,bc", BCS)
 @pytest.mark.parametrize("case", [0, 1, 2])
-def test_jacobi_sweep_fixed_point_is_fv_solution(heatgold, case):
-    """Every boundary formula of the reference sweep (faces, edges, vertices)
-    leaves the finite-volume steady state unchanged."""
-    p, nzh = _heat_params(heatgold[f"spec{case}"])
-    ref = heatgold[f"T{case}"]
-    Tf = O.jacobi_sweep(p, nzh, ref, heatgold[f"Q{case}"])
+def test_jacobi_sweep_fixed_point_is_fv_solution(heatgold, case, pre, bc):
+    """Every boundary formula of the sweep (faces, edges, vertices) leaves the
+    finite-volume steady state unchanged."""
+    p, nzh = _heat_params(heatgold[f"{pre}spec{case}"], bc)
+    ref = heatgold[f"{pre}T{case}"]
+    Tf = O.jacobi_sweep(p, nzh, ref, heatgold[f"{pre}Q{case}"])
     assert np.abs(Tf - ref).max() < 1e-9 * np.abs(ref).max()
 
 
-def test_jacobi_iteration_converges_to_direct(heatgold):
-    p, nzh = _heat_params(heatgold["spec1"])
-    Q = heatgold["Q1"]
+def test_oven_boundary_differs_from_open_top(heatgold):
+    """The two boundary sets give different steady states for the same
+    source (a restatement that ignored heat_bc would pass the tests above
+    only by accident of the fixtures)."""
+    spec, Q = heatgold["ospec0"], heatgold["oQ0"]
+    To = O.heat_solve_direct(_heat_params(spec, abi.HEAT_BC_OVEN_SURROUNDED)[0], int(spec[2]), Q)
+    Tt = O.heat_solve_direct(_heat_params(spec, abi.HEAT_BC_OPEN_TOP)[0], int(spec[2]), Q)
+    assert np.abs(To - Tt).max() > 1e-3 * np.abs(To - spec[9]).max()
+    # the four lateral planes sit at the oven temperature
+    assert np.all(To[:, :, 0] == spec[9]) and np.all(To[:, :, -1] == spec[9])
+    assert np.all(To[:, 0, :] == spec[9]) and np.all(To[:, -1, :] == spec[9])
+
+
+@pytest.mark.parametrize("pre,bc", BCS)
+def test_jacobi_iteration_converges_to_direct(heatgold, pre, bc):
+    p, nzh = _heat_params(heatgold[f"{pre}spec1"], bc)
+    Q = heatgold[f"{pre}Q1"]
     T = np.full_like(Q, p.T_peltier_C)
     for _ in range(20000):
         Tn = O.jacobi_sweep(p, nzh, T, Q)
         if np.abs(Tn - T).max() < 1e-13:
             break
         T = Tn
-    assert np.abs(T - heatgold["T1"]).max() < 1e-9
+    assert np.abs(T - heatgold[f"{pre}T1"]).max() < 1e-9
 
 
 def test_thermal_uniform_equals_plain():
