@@ -35,6 +35,7 @@ SCHEME_STRANG = 1
 PREC_FP32_MIXED = 0
 PREC_FP64_PLAIN = 1
 PREC_FP32_STORAGE = 2
+PREC_FP64_FUSED = 3
 HEAT_JACOBI_REFERENCE = 0
 HEAT_DIRECT = 1
 HEAT_BC_OPEN_TOP = 0

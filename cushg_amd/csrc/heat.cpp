@@ -265,7 +265,7 @@ size_t heat_bytes(const shg_params* p) {
     const size_t n3 = (size_t)p->nx * p->ny * nzh;
     const HeatCoef c = heat_coef(p, (int)nzh);
     const size_t nu = (size_t)c.nxu * c.nyu * nzh;
-    const size_t rs = p->precision == SHG_PREC_FP64_PLAIN ? 8 : 4;
+    const size_t rs = (p->precision == SHG_PREC_FP64_PLAIN || p->precision == SHG_PREC_FP64_FUSED) ? 8 : 4;
     size_t b = 4 * al256(8 * n3) + al256(8 * n3) + al256(4 * n3);  // T, Tn, dk, dkbase; nn, Q
     b += 2 * al256(rs * nu);                           // u, v
     const size_t nb = (size_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * nzh;
@@ -295,7 +295,7 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     const size_t n3 = (size_t)hs.nx * hs.ny * hs.nzh;
     const HeatCoef c = heat_coef(p, hs.nzh);
     const size_t nu = (size_t)c.nxu * c.nyu * hs.nzh;
-    const bool f64 = p->precision == SHG_PREC_FP64_PLAIN;
+    const bool f64 = p->precision == SHG_PREC_FP64_PLAIN || p->precision == SHG_PREC_FP64_FUSED;
     Arena& A = h->arena;
     hs.T = A.carve<double>(n3);
     hs.Tn = A.carve<double>(n3);

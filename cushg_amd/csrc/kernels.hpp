@@ -5,17 +5,26 @@
 
 namespace shg {
 
-// One instance of a fused march (both FP32-field forms), as the kernels
-// see it.  Lives in device memory (an array per launch group, indexed by
-// blockIdx.z).  Constants are FP64; the FP32-arithmetic kernels round them
-// once at use.
+// Storage / arithmetic pair of the fused passes.
+enum class Arith {
+    F32,  // FP32 fields, FP32 arithmetic          (SHG_PREC_FP32_MIXED)
+    S32,  // FP32 fields, FP64 arithmetic          (SHG_PREC_FP32_STORAGE)
+    F64,  // FP64 fields, FP64 arithmetic          (SHG_PREC_FP64_FUSED)
+};
+inline size_t arith_field_bytes(Arith a) { return a == Arith::F64 ? 16 : 8; }
+
+// One instance of a fused march, as the kernels see it.  Lives in device
+// memory (an array per launch group, indexed by blockIdx.z).  Field
+// pointers point to float2 or double2 per the launch's form.  Constants
+// are FP64; the FP32-arithmetic kernels round them once at use.
 struct MixedInst {
-    float2* P;              // pump field [ny][nx]
-    float2* S;              // SH field   [ny][nx]
-    float2* outP;           // state after nz-1 slices (the reference's Pump_out)
-    float2* outS;
-    float2* trP;            // per-slice trace: state before slice g's SSFM step (or null)
-    float2* trS;
+    void* P;                // pump field [ny][nx]
+    void* S;                // SH field   [ny][nx]
+    void* outP;             // state after nz-1 slices (the reference's Pump_out)
+    void* outS;
+    void* trP;              // per-slice trace: state before slice g's SSFM step (or null)
+    void* trS;
+    void* hist;             // column-pass outputs of slices [hist_g0, hist_g0 + hist_n) (or null)
     const double2* exP;     // separable propagator, x factor [nx] (includes 1/(nx ny))
     const double2* exS;
     const double2* eyP;     // y factor [ny]
@@ -37,15 +46,16 @@ struct MixedInst {
     int nx, ny, nz, nzh;
     int crystal;
     int trace_every;
+    int hist_g0, hist_n;
 };
 
-// fused passes (kernels_mixed.hip); fp64: double2 arithmetic in registers
-hipError_t launch_rowpass(int nx, bool thermal, bool fp64, int rows_pw, dim3 grid, hipStream_t st,
+// fused passes (kernels_mixed*.hip)
+hipError_t launch_rowpass(int nx, bool thermal, Arith a, int rows_pw, dim3 grid, hipStream_t st,
                           const MixedInst* insts, int g, const double2* tw);
-hipError_t launch_colpass(int ny, bool fp64, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
+hipError_t launch_colpass(int ny, Arith a, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
                           int g, const double2* tw);
-int rowpass_rows_default(int nx, bool fp64);
-int colpass_cols_default(int ny, bool fp64);
+int rowpass_rows_default(int nx, Arith a);
+int colpass_cols_default(int ny, Arith a);
 bool colpass_cols_supported(int ny, int cols);
 bool rowpass_rows_supported(int nx, int rows);
 

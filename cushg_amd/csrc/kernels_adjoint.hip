@@ -1,0 +1,442 @@
+// kernels_adjoint.hip -- reverse-mode (adjoint) passes of the fused
+// non-THERMAL march, FP64 arithmetic, fields stored as S (float2 or
+// double2).  No reference counterpart: the reference computes no gradients.
+//
+// Forward slice s on the spatial state X_s (notation of the fused passes):
+//   U_s = R1_s(X_s);  V_s = Fx U_s;  C_s = IFy(Pi . Fy V_s);
+//   Y_s = IFx C_s;    X_{s+1} = R2_s(Y_s)
+// (R: RK4 calls, F/IF: unnormalised forward/inverse DFT along x or y, Pi:
+// propagator with 1/(nx ny)).  For a real scalar L and the real inner
+// product Re<a, b>, the gradients (Wirtinger form, grad = dL/dRe + i dL/dIm)
+// run backwards through
+//   lam_Y = R2_s^T lam_X(s+1);  lam_C = Fx lam_Y;
+//   lam_V = IFy(conj(Pi) . Fy lam_C);  lam_U = IFx lam_V;  lam_X = R1_s^T lam_U
+// (IF^T = F, F^T = IF for the unnormalised transforms), grouped like the
+// forward passes into an adjoint row pass (IFx -> R1_s^T -> R2_{s-1}^T -> Fx)
+// and an adjoint column pass (Fy -> conj(Pi) -> IFy).  The row pass rebuilds
+// Y_{s-1} = IFx C_{s-1} and X_s = R2_{s-1}(Y_{s-1}) from the stored
+// column-pass output C_{s-1}.
+#include "adjoint.hpp"
+#include "mixed_passes.hpp"
+
+namespace shg {
+
+// Reverse of dA/dz (src/headers/Solver.h:57-80) for gradients GP, GS of
+// dP, dS:  returns grad_P, grad_S and grad_e, and accumulates dL/dkp,
+// dL/dks, dL/dbfac.
+//   dP = i kp S P* e* - aph P        dS = i ks P^2 e - (ash + bfac |S|^2 / 2) S
+__device__ __forceinline__ void dadz_adj(double2 P, double2 S, double2 e, double kp, double ks, double aph,
+                                         double ash, double bfac, double2 GP, double2 GS, double2& gP,
+                                         double2& gS, double2& ge, double& gkp, double& gks, double& gb) {
+    const double2 ec = conjc(e), Pc = conjc(P);
+    const double2 GPc = conjc(GP), GSc = conjc(GS);
+    // grad_P = -aph GP + i kp S e* conj(GP) - 2 i ks P* e* GS
+    const double2 t1 = cmul(cmul(S, ec), GPc);
+    const double2 t2 = cmul(cmul(Pc, ec), GS);
+    gP = mk<double2>(-aph * GP.x - kp * t1.y + 2.0 * ks * t2.y, -aph * GP.y + kp * t1.x - 2.0 * ks * t2.x);
+    // grad_S = -i kp P e GP - (ash + bfac |S|^2) GS - bfac/2 S^2 conj(GS)
+    const double2 t3 = cmul(cmul(P, e), GP);
+    const double s2 = abs2(S);
+    const double2 t4 = cmul(cmul(S, S), GSc);
+    const double g = ash + bfac * s2;
+    gS = mk<double2>(kp * t3.y - g * GS.x - 0.5 * bfac * t4.x, -kp * t3.x - g * GS.y - 0.5 * bfac * t4.y);
+    // grad_e = i kp S P* conj(GP) - i ks P*^2 GS
+    const double2 t5 = cmul(cmul(S, Pc), GPc);
+    const double2 t6 = cmul(cmul(Pc, Pc), GS);
+    ge = mk<double2>(-kp * t5.y + ks * t6.y, kp * t5.x - ks * t6.x);
+    // dL/dkp = Re(conj(GP) i S P* e*), dL/dks = Re(conj(GS) i P^2 e), dL/dbfac = Re(conj(GS)(-|S|^2 S / 2))
+    const double2 u = cmul(cmul(S, Pc), ec);
+    gkp += GP.y * u.x - GP.x * u.y;
+    const double2 w = cmul(cmul(P, P), e);
+    gks += GS.y * w.x - GS.x * w.y;
+    gb += -0.5 * s2 * (GS.x * S.x + GS.y * S.y);
+}
+
+// Reverse of one RK4 call (src/headers/Solver.h:231-257) at forward input
+// (P, S): lam <- J^T lam.  acc[0..2] += dL/dphi_j of the three phase factors
+// (e_j = exp(i phi_j): dL/dphi_j = Re(conj(grad_e_j) i e_j)), acc[3..5] +=
+// dL/dkp, dL/dks, dL/dbfac.
+__device__ __forceinline__ void rk4_adj(double2& lP, double2& lS, double2 P, double2 S, const double2 (&e)[3],
+                                        double kp, double ks, double bfac, const RkConst<double>& k,
+                                        double (&acc)[6]) {
+    double2 k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
+    dadz(P, S, e[0], kp, ks, k.aph, k.ash, bfac, k1p, k1s);
+    const double2 u2p = P + k.c1 * k1p, u2s = S + k.c1 * k1s;
+    dadz(u2p, u2s, e[1], kp, ks, k.aph, k.ash, bfac, k2p, k2s);
+    const double2 u3p = P + k.c2 * k2p, u3s = S + k.c2 * k2s;
+    dadz(u3p, u3s, e[1], kp, ks, k.aph, k.ash, bfac, k3p, k3s);
+    const double2 u4p = P + k.c3 * k3p, u4s = S + k.c3 * k3s;
+    (void)k4p;
+    (void)k4s;
+    double2 aP = lP, aS = lS;
+    double2 a4p = k.w * lP, a4s = k.w * lS;
+    double2 a3p = (2.0 * k.w) * lP, a3s = (2.0 * k.w) * lS;
+    double2 a2p = a3p, a2s = a3s;
+    double2 a1p = a4p, a1s = a4s;
+    double2 gP, gS, ge;
+    double2 ge1 = mk<double2>(0.0, 0.0);
+    dadz_adj(u4p, u4s, e[2], kp, ks, k.aph, k.ash, bfac, a4p, a4s, gP, gS, ge, acc[3], acc[4], acc[5]);
+    aP = aP + gP;
+    aS = aS + gS;
+    a3p = a3p + k.c3 * gP;
+    a3s = a3s + k.c3 * gS;
+    const double2 ge2 = ge;
+    dadz_adj(u3p, u3s, e[1], kp, ks, k.aph, k.ash, bfac, a3p, a3s, gP, gS, ge, acc[3], acc[4], acc[5]);
+    aP = aP + gP;
+    aS = aS + gS;
+    a2p = a2p + k.c2 * gP;
+    a2s = a2s + k.c2 * gS;
+    ge1 = ge1 + ge;
+    dadz_adj(u2p, u2s, e[1], kp, ks, k.aph, k.ash, bfac, a2p, a2s, gP, gS, ge, acc[3], acc[4], acc[5]);
+    aP = aP + gP;
+    aS = aS + gS;
+    a1p = a1p + k.c1 * gP;
+    a1s = a1s + k.c1 * gS;
+    ge1 = ge1 + ge;
+    dadz_adj(P, S, e[0], kp, ks, k.aph, k.ash, bfac, a1p, a1s, gP, gS, ge, acc[3], acc[4], acc[5]);
+    aP = aP + gP;
+    aS = aS + gS;
+    const double2 ge0 = ge;
+    // dL/dphi = Re(conj(g) i e) = g.x * (-e.y) + g.y * e.x
+    acc[0] += -ge0.x * e[0].y + ge0.y * e[0].x;
+    acc[1] += -ge1.x * e[1].y + ge1.y * e[1].x;
+    acc[2] += -ge2.x * e[2].y + ge2.y * e[2].x;
+    lP = aP;
+    lS = aS;
+}
+
+// block sum of NV values per thread into part[blockIdx.x * NV + v]
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* part, double* red) {
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nt = blockDim.x * blockDim.y;
+    const int nw = (nt + 63) / 64;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double a = v[k];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+        if ((t & 63) == 0) red[(t >> 6) * NV + k] = a;
+    }
+    __syncthreads();
+    if (t < NV) {
+        double a = 0.0;
+        for (int w = 0; w < nw; w++) a += red[w * NV + t];
+        part[(size_t)(blockIdx.x + gridDim.x * blockIdx.y) * NV + t] = a;
+    }
+}
+
+// Adjoint row pass at boundary s: rows of ROWS per workgroup, NX/8 threads
+// per row; grid (ny/ROWS).  Partial sums per workgroup (12):
+//   [0..5]  call R1_s:     dL/dphi_0..2, dL/dkp, dL/dks, dL/dbfac
+//   [6..11] call R2_{s-1}: the same
+template <int NX, int ROWS, typename S>
+__global__ __launch_bounds__(NX / 8 * ROWS) void k_adj_row(AdjArgs a, const double2* __restrict__ tw) {
+    using C = double2;
+    constexpr int TPR = NX / 8;
+    constexpr int LD = FftShape<NX>::LDS;
+    __shared__ C lds[ROWS * LD];
+    __shared__ double red[16 * 12];
+    const MixedInst& I = *a.I;
+    const int y = blockIdx.x * ROWS + threadIdx.y;
+    const int tp = threadIdx.x;
+    C* my = lds + threadIdx.y * LD;
+    const size_t base = (size_t)y * NX + tp;
+    C lf[2][8], yf[2][8];
+    const S* fP = static_cast<const S*>(a.fwdP);
+    const S* fS = static_cast<const S*>(a.fwdS);
+    S* LP = static_cast<S*>(a.lamP);
+    S* LS = static_cast<S*>(a.lamS);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        lf[0][q] = from_store<C>(LP[base + q * TPR]);
+        lf[1][q] = from_store<C>(LS[base + q * TPR]);
+        yf[0][q] = from_store<C>(fP[base + q * TPR]);
+        yf[1][q] = from_store<C>(fS[base + q * TPR]);
+    }
+    if (a.first) {
+        fft_dist<C, NX, true>(lf[0], my, tw, tp);
+        fft_dist<C, NX, true>(lf[1], my, tw, tp);
+    }
+    if (!a.fwd_spatial) {
+        fft_dist<C, NX, true>(yf[0], my, tw, tp);
+        fft_dist<C, NX, true>(yf[1], my, tw, tp);
+    }
+    const RkConst<double> k{I.c1, I.c2, I.c3, I.w, I.ap_h, I.as_h};
+    const int s = a.s;
+    C e1[3], e2[3];
+    {
+        const double2* t1 = I.phase + (size_t)(s * 2 + 0) * 3;            // R1_s
+        const double2* t2 = I.phase + (size_t)(max(s - 1, 0) * 2 + 1) * 3;  // R2_{s-1}
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            e1[j] = t1[j];
+            e2[j] = t2[j];
+        }
+    }
+    double acc1[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const C Yp = yf[0][q], Ys = yf[1][q];
+        C Xp = Yp, Xs = Ys;
+        if (!a.fwd_spatial) rk4_point(Xp, Xs, e2, I.kp, I.ks, I.bfac, k);  // X_s = R2_{s-1}(Y_{s-1})
+        C lp = lf[0][q], ls = lf[1][q];
+        if (a.first) rk4_adj(lp, ls, Xp, Xs, e1, I.kp, I.ks, I.bfac, k, acc1);
+        if (a.second) rk4_adj(lp, ls, Yp, Ys, e2, I.kp, I.ks, I.bfac, k, acc2);
+        lf[0][q] = lp;
+        lf[1][q] = ls;
+    }
+    if (a.second) {
+        fft_dist<C, NX, false>(lf[0], my, tw, tp);
+        fft_dist<C, NX, false>(lf[1], my, tw, tp);
+    }
+    const uint32_t kp = store_key(4, 0, s), ks = store_key(4, 1, s);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const size_t i = base + q * TPR;
+        LP[i] = to_store<S>(lf[0][q], i, kp);
+        LS[i] = to_store<S>(lf[1][q], i, ks);
+    }
+    double v[12];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        v[j] = acc1[j];
+        v[6 + j] = acc2[j];
+    }
+    block_sums<12>(v, a.part, red);
+}
+
+// Adjoint column pass of slice s: COLS columns of one field per workgroup
+// (blockIdx.y: 0 pump, 1 SH); grid (nx/COLS, 2).  Partial sums per
+// workgroup, [pump, SH], of  sum_m Re(conj(Fy lam_C)_m i (Fy C_s)_m)
+// (fx_m^2 + fy_m^2): the propagator-wavenumber gradient up to the factor
+// dz 2 pi^2 / (kw^2 ny).
+template <int NY, int COLS, typename S>
+__global__ __launch_bounds__(NY / 8 * COLS) void k_adj_col(AdjArgs a, const double2* __restrict__ tw) {
+    using C = double2;
+    constexpr int TPC = NY / 8;
+    constexpr int LD = FftShape<NY>::LDS;
+    __shared__ C lds[COLS * LD];
+    __shared__ double red[16 * 2];
+    const MixedInst& I = *a.I;
+    const int nx = I.nx;
+    const int c = threadIdx.x, tp = threadIdx.y;
+    const int x = blockIdx.x * COLS + c;
+    const bool sh = blockIdx.y != 0;
+    S* L = static_cast<S*>(sh ? a.lamS : a.lamP);
+    const S* F = static_cast<const S*>(sh ? a.fwdS : a.fwdP);
+    const double2* ex = sh ? I.exS : I.exP;
+    const double2* ey = sh ? I.eyS : I.eyP;
+    C* my = lds + c * LD;
+    C v[8], w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        v[q] = from_store<C>(L[(size_t)(tp + q * TPC) * nx + x]);
+        w[q] = from_store<C>(F[(size_t)(tp + q * TPC) * nx + x]);
+    }
+    fft_dist<C, NY, false>(v, my, tw, tp);
+    fft_dist<C, NY, false>(w, my, tw, tp);
+    const int ic = (x + nx / 2) % nx;
+    const double fx = (ic - 0.5 * nx) * a.dfx;
+    const double exc = ex[x].x, exs = -ex[x].y;  // conj(ex)
+    double gk = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int ky = tp + q * TPC;
+        const int jc = (ky + NY / 2) % NY;
+        const double fy = (jc - 0.5 * NY) * a.dfy;
+        // Re(conj(v) i w) = v.x * (-w.y) + v.y * w.x
+        gk += (-v[q].x * w[q].y + v[q].y * w[q].x) * (fx * fx + fy * fy);
+        const double2 pr = cmul(mk<double2>(exc, exs), conjc(ey[ky]));
+        v[q] = cmul(v[q], pr);
+    }
+    fft_dist<C, NY, true>(v, my, tw, tp);
+    const uint32_t key = store_key(5, sh ? 1 : 0, a.s);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const size_t i = (size_t)(tp + q * TPC) * nx + x;
+        L[i] = to_store<S>(v[q], i, key);
+    }
+    double vv[2] = {sh ? 0.0 : gk, sh ? gk : 0.0};
+    block_sums<2>(vv, a.part, red);
+}
+
+// Seed: lam_P = 0, lam_S = coef * S_out (the gradient of
+// eta = c_s sum |S|^2 / P with respect to the saved SH field).
+template <typename S>
+__global__ void k_adj_seed(S* __restrict__ lp, S* __restrict__ ls, const S* __restrict__ sout, long n, double coef) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
+        const S v = sout[i];
+        S z;
+        z.x = 0;
+        z.y = 0;
+        lp[i] = z;
+        S o;
+        o.x = coef * v.x;
+        o.y = coef * v.y;
+        ls[i] = o;
+    }
+}
+
+// Initial-field gradients: with lam the gradient at the pump's initial
+// field A0 (src/headers/Efields.h:84-119),  sum Re(conj(lam) A0 D)  for
+// D = d ln A0 / d waist, d ln A0 / d focus, d ln A0 / d n_p, where
+//   A0 = (Ap0 / M) exp(-r^2 / (w0^2 M)),  M = 1 - i f / zR,
+//   zR = pi n w0^2 / l,  Ap0 ~ (n w0^2)^(-1/2).
+// grid-stride over rows, block (256); partials [block][3].
+template <typename S>
+__global__ __launch_bounds__(256) void k_adj_pump(const S* __restrict__ lam, int nx, int ny, double dx, double dy,
+                                                  double w0, double f, double n, double zR, double Ap0,
+                                                  double* __restrict__ part) {
+    __shared__ double red[4 * 3];
+    const double2 M = make_double2(1.0, -f / zR);
+    const double2 Mi = make_double2(M.x / abs2(M), -M.y / abs2(M));  // 1/M
+    const double2 Mi2 = cmul(Mi, Mi);
+    const double w2 = w0 * w0;
+    // dM/dw0 = -2 (M - 1) / w0 ; dM/df = -i / zR ; dM/dn = -(M - 1) / n
+    const double2 Mm1 = make_double2(M.x - 1.0, M.y);
+    const double2 dMw = make_double2(-2.0 * Mm1.x / w0, -2.0 * Mm1.y / w0);
+    const double2 dMf = make_double2(0.0, -1.0 / zR);
+    const double2 dMn = make_double2(-Mm1.x / n, -Mm1.y / n);
+    double acc[3] = {0, 0, 0};
+    const long N = (long)nx * ny;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < N; i += 256L * gridDim.x) {
+        const int x = (int)(i % nx), y = (int)(i / nx);
+        const double rx = (x - 0.5 * nx) * dx, ry = (y - 0.5 * ny) * dy;
+        const double r2 = rx * rx + ry * ry;
+        // A0 = Ap0 Mi exp(-r2 Mi / w2)
+        const double2 ex = make_double2(-r2 * Mi.x / w2, -r2 * Mi.y / w2);
+        const double m = exp(ex.x);
+        const double2 A0 = cmul(make_double2(Ap0 * Mi.x, Ap0 * Mi.y), make_double2(m * cos(ex.y), m * sin(ex.y)));
+        // d ln A0 = d ln Ap0 - dM / M + r2 [ d(w2) / (w2^2 M) + dM / (w2 M^2) ]
+        const double2 dMw_M = cmul(dMw, Mi), dMf_M = cmul(dMf, Mi), dMn_M = cmul(dMn, Mi);
+        const double2 dMw_M2 = cmul(dMw, Mi2), dMf_M2 = cmul(dMf, Mi2), dMn_M2 = cmul(dMn, Mi2);
+        const double2 Dw = make_double2(-1.0 / w0 - dMw_M.x + r2 * (2.0 * w0 * Mi.x / (w2 * w2) + dMw_M2.x / w2),
+                                        -dMw_M.y + r2 * (2.0 * w0 * Mi.y / (w2 * w2) + dMw_M2.y / w2));
+        const double2 Df = make_double2(-dMf_M.x + r2 * dMf_M2.x / w2, -dMf_M.y + r2 * dMf_M2.y / w2);
+        const double2 Dn = make_double2(-0.5 / n - dMn_M.x + r2 * dMn_M2.x / w2, -dMn_M.y + r2 * dMn_M2.y / w2);
+        const S lv = lam[i];
+        const double2 l = make_double2(lv.x, lv.y);
+        const double2 aw = cmul(A0, Dw), af = cmul(A0, Df), an = cmul(A0, Dn);
+        acc[0] += l.x * aw.x + l.y * aw.y;
+        acc[1] += l.x * af.x + l.y * af.y;
+        acc[2] += l.x * an.x + l.y * an.y;
+    }
+    block_sums<3>(acc, part, red);
+}
+
+// out[r * nv + v] = sum_b part[(r * nb + b) * nv + v] for every row r
+// (one workgroup per row; deterministic order)
+__global__ __launch_bounds__(256) void k_adj_reduce(const double* __restrict__ part, int nb, int nv,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    const double* pr = part + (size_t)blockIdx.x * nb * nv;
+    for (int v = 0; v < nv; v++) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) a += pr[(size_t)b * nv + v];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * nv + v] = red[0];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------
+template <int NX> constexpr int adj_rows() { return NX / 8 >= 64 ? 1 : 64 / (NX / 8); }
+
+template <typename S>
+static hipError_t adj_row_n(int nx, dim3 grid, hipStream_t st, const AdjArgs& a, const double2* tw) {
+    switch (nx) {
+#define SHG_ADJROW(N)                                                                                 \
+    case N:                                                                                           \
+        hipLaunchKernelGGL((k_adj_row<N, adj_rows<N>(), S>), grid, dim3(N / 8, adj_rows<N>()), 0, st, a, tw); \
+        return hipGetLastError();
+        SHG_ADJROW(64)
+        SHG_ADJROW(128)
+        SHG_ADJROW(192)
+        SHG_ADJROW(256)
+        SHG_ADJROW(384)
+        SHG_ADJROW(512)
+#undef SHG_ADJROW
+    }
+    return hipErrorInvalidValue;
+}
+
+int adj_row_workgroups(int nx, int ny) {
+    int rows = 1;
+    switch (nx) {
+        case 64: rows = adj_rows<64>(); break;
+        case 128: rows = adj_rows<128>(); break;
+        case 192: rows = adj_rows<192>(); break;
+        case 256: rows = adj_rows<256>(); break;
+        case 384: rows = adj_rows<384>(); break;
+        case 512: rows = adj_rows<512>(); break;
+    }
+    return (ny + rows - 1) / rows;
+}
+
+hipError_t launch_adj_row(bool f64, int nx, int ny, hipStream_t st, const AdjArgs& a, const double2* tw) {
+    const dim3 grid(adj_row_workgroups(nx, ny));
+    return f64 ? adj_row_n<double2>(nx, grid, st, a, tw) : adj_row_n<float2>(nx, grid, st, a, tw);
+}
+
+template <typename S>
+static hipError_t adj_col_n(int ny, int nx, hipStream_t st, const AdjArgs& a, const double2* tw) {
+    constexpr int COLS = 8;
+    const dim3 grid(nx / COLS, 2);
+    switch (ny) {
+#define SHG_ADJCOL(N)                                                                                 \
+    case N:                                                                                           \
+        hipLaunchKernelGGL((k_adj_col<N, COLS, S>), grid, dim3(COLS, N / 8), 0, st, a, tw);          \
+        return hipGetLastError();
+        SHG_ADJCOL(64)
+        SHG_ADJCOL(128)
+        SHG_ADJCOL(192)
+        SHG_ADJCOL(256)
+        SHG_ADJCOL(384)
+        SHG_ADJCOL(512)
+#undef SHG_ADJCOL
+    }
+    return hipErrorInvalidValue;
+}
+
+int adj_col_workgroups(int nx) { return nx / 8 * 2; }
+
+hipError_t launch_adj_col(bool f64, int nx, int ny, hipStream_t st, const AdjArgs& a, const double2* tw) {
+    return f64 ? adj_col_n<double2>(ny, nx, st, a, tw) : adj_col_n<float2>(ny, nx, st, a, tw);
+}
+
+hipError_t launch_adj_seed(bool f64, void* lp, void* ls, const void* sout, long n, double coef, hipStream_t st) {
+    if (f64)
+        hipLaunchKernelGGL(k_adj_seed<double2>, dim3(1024), dim3(256), 0, st, static_cast<double2*>(lp),
+                           static_cast<double2*>(ls), static_cast<const double2*>(sout), n, coef);
+    else
+        hipLaunchKernelGGL(k_adj_seed<float2>, dim3(1024), dim3(256), 0, st, static_cast<float2*>(lp),
+                           static_cast<float2*>(ls), static_cast<const float2*>(sout), n, coef);
+    return hipGetLastError();
+}
+
+int adj_pump_workgroups() { return 512; }
+
+hipError_t launch_adj_pump(bool f64, const void* lam, int nx, int ny, double dx, double dy, double w0, double f,
+                           double n, double zR, double Ap0, double* part, hipStream_t st) {
+    if (f64)
+        hipLaunchKernelGGL(k_adj_pump<double2>, dim3(adj_pump_workgroups()), dim3(256), 0, st,
+                           static_cast<const double2*>(lam), nx, ny, dx, dy, w0, f, n, zR, Ap0, part);
+    else
+        hipLaunchKernelGGL(k_adj_pump<float2>, dim3(adj_pump_workgroups()), dim3(256), 0, st,
+                           static_cast<const float2*>(lam), nx, ny, dx, dy, w0, f, n, zR, Ap0, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_reduce(const double* part, int rows, int nb, int nv, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_reduce, dim3(rows), dim3(256), 0, st, part, nb, nv, out);
+    return hipGetLastError();
+}
+
+}  // namespace shg

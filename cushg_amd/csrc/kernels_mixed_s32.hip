@@ -1,0 +1,16 @@
+// kernels_mixed_s32.hip -- fused passes with FP32 fields, FP64 arithmetic.
+#include "mixed_passes.hpp"
+
+namespace shg {
+
+hipError_t rowpass_s32(int nx, bool thermal, int rows, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
+                        const double2* tw) {
+    return rowpass_dispatch<float2, double2>(nx, thermal, rows, grid, st, insts, g, tw);
+}
+
+hipError_t colpass_s32(int ny, int cols, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
+                        const double2* tw) {
+    return colpass_dispatch<float2, double2>(ny, cols, grid, st, insts, g, tw);
+}
+
+}  // namespace shg

@@ -1,6 +1,7 @@
-// march_mixed.cpp -- host orchestration of the FP32-mixed form: instance
-// set-up, the batched z-march (Solver::CWES, src/headers/Solver.h:276-308)
-// and, with THERMAL, the thermo-optic outer loop (src/headers/Solver.h:321-373).
+// march_mixed.cpp -- host orchestration of the fused forms (FP32 fields
+// with FP32 or FP64 arithmetic, FP64 fields): instance set-up, the batched
+// z-march (Solver::CWES, src/headers/Solver.h:276-308) and, with THERMAL,
+// the thermo-optic outer loop (src/headers/Solver.h:321-373).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,62 +15,70 @@
 #include "common.hpp"
 #include "heat.hpp"
 #include "kernels.hpp"
+#include "march_mixed.hpp"
 #include "runtime.hpp"
 
 namespace shg {
 
-struct MixedHost {
-    const shg_params* p = nullptr;
-    shg_result* r = nullptr;
-    int nx = 0, ny = 0, nz = 0, nzh = 0;
-    size_t N2 = 0;
-    double dx = 0, dy = 0, dz = 0;
-    double Tc = 0;  // temperature at the grid centre of heat slice 0
-    float2 *P = nullptr, *S = nullptr, *outP = nullptr, *outS = nullptr;
-    SliceTrace* trace = nullptr;  // per-slice trace (single runs)
-    float2 *trP = nullptr, *trS = nullptr;
-    int pass = 0;                 // thermo-optic pass being marched
-    double2 *exP = nullptr, *exS = nullptr, *eyP = nullptr, *eyS = nullptr;
-    double2* phase = nullptr;
-    double* pol = nullptr;
-    double* slots = nullptr;  // [0] P_in, [1] P_out, [2] P_sh
-    HeatState heat;
-    MixedInst desc{};
-    bool thermal = false;
-    bool fp64 = false;    // SHG_PREC_FP32_STORAGE: FP64 arithmetic in the fused passes
-    bool running = true;  // THERMAL: still in the outer loop
-    int outer = 0;
-    long long steps = 0, sweeps = 0;
-    double heat_res = 0;
-};
+Arith arith_of(int precision) {
+    switch (precision) {
+        case SHG_PREC_FP32_STORAGE: return Arith::S32;
+        case SHG_PREC_FP64_FUSED: return Arith::F64;
+        default: return Arith::F32;
+    }
+}
 
-static const char* rowpass_name(int nx, bool th, bool fp64) {
-    static std::map<std::tuple<int, bool, bool>, std::string> names;
-    auto& s = names[{nx, th, fp64}];
-    if (s.empty()) s = std::string("rowpass_") + std::to_string(nx) + (th ? "_thermal" : "") + (fp64 ? "_f64" : "");
+static const char* arith_suffix(Arith a) {
+    return a == Arith::F32 ? "" : a == Arith::S32 ? "_f64" : "_fp64fields";
+}
+const char* rowpass_name(int nx, bool th, Arith a) {
+    static std::map<std::tuple<int, bool, int>, std::string> names;
+    auto& s = names[{nx, th, (int)a}];
+    if (s.empty()) s = std::string("rowpass_") + std::to_string(nx) + (th ? "_thermal" : "") + arith_suffix(a);
     return s.c_str();
 }
-static const char* colpass_name(int ny, bool fp64) {
-    static std::map<std::pair<int, bool>, std::string> names;
-    auto& s = names[{ny, fp64}];
-    if (s.empty()) s = "colpass_" + std::to_string(ny) + (fp64 ? "_f64" : "");
+const char* colpass_name(int ny, Arith a) {
+    static std::map<std::pair<int, int>, std::string> names;
+    auto& s = names[{ny, (int)a}];
+    if (s.empty()) s = "colpass_" + std::to_string(ny) + arith_suffix(a);
     return s.c_str();
 }
 
-int plan_rows(shg_handle* h, int nx, bool fp64) {
-    auto it = h->rows_plan.find({nx, fp64});
-    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx, fp64);
+int plan_rows(shg_handle* h, int nx, Arith a) {
+    auto it = h->rows_plan.find({nx, (int)a});
+    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx, a);
 }
-int plan_cols(shg_handle* h, int ny, bool fp64) {
-    auto it = h->cols_plan.find({ny, fp64});
-    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_default(ny, fp64);
+int plan_cols(shg_handle* h, int ny, Arith a) {
+    auto it = h->cols_plan.find({ny, (int)a});
+    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_default(ny, a);
+}
+
+int field_sum_sq(shg_handle* h, Arith a, const void* A, long n, double* slot) {
+    return a == Arith::F64 ? sum_sq64(h, static_cast<const double2*>(A), n, slot)
+                           : sum_sq(h, static_cast<const float2*>(A), n, slot);
+}
+
+int field_to_host(shg_handle* h, Arith a, const void* A, size_t n, double* out) {
+    if (a == Arith::F64) {
+        SHG_HIP(hipMemcpyAsync(out, A, sizeof(double2) * n, hipMemcpyDeviceToHost, h->stream));
+        SHG_HIP(hipStreamSynchronize(h->stream));
+        return SHG_OK;
+    }
+    std::vector<float2> buf(n);
+    SHG_HIP(hipMemcpyAsync(buf.data(), A, sizeof(float2) * n, hipMemcpyDeviceToHost, h->stream));
+    SHG_HIP(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; i++) {
+        out[2 * i] = buf[i].x;
+        out[2 * i + 1] = buf[i].y;
+    }
+    return SHG_OK;
 }
 
 // Host-side FP64 tables of one instance at centre temperature Tc:
 // separable propagator (src/headers/Efields.h:122-141 with the fftshift of
 // :68-81 folded into the index map, and the 1/(nx ny) of cuFFTscale folded
 // into the x factor), per-slice QPM phases, couplings and RK4 constants.
-static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>& ex_s,
+void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>& ex_s,
                         std::vector<double2>& ey_p, std::vector<double2>& ey_s,
                         std::vector<double2>& phase) {
     const shg_params* p = m.p;
@@ -160,7 +169,7 @@ static void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<do
 
 // Upload tables, initialise the pump (src/headers/Efields.h:84-119) and a
 // zero SH field, and reduce the initial pump power.
-static int setup_fields(shg_handle* h, MixedHost& m) {
+int setup_fields(shg_handle* h, MixedHost& m) {
     std::vector<double2> exp_, exs, eyp, eys;
     std::vector<double2> ph;
     host_tables(m, exp_, exs, eyp, eys, ph);
@@ -179,29 +188,25 @@ static int setup_fields(shg_handle* h, MixedHost& m) {
     const double Ap0 = std::sqrt(4.0 * p->power_W / (kEps0 * kC * kPi * np * w2));
     {
         ProfScope ps(h, "pump_init");
-        SHG_HIP(launch_pump_init<float2>(h->stream, m.P, m.nx, m.ny, m.dx, m.dy, w2, Ap0, 1.0, -p->focal_um / zR));
+        if (m.arith == Arith::F64)
+            SHG_HIP(launch_pump_init<double2>(h->stream, static_cast<double2*>(m.P), m.nx, m.ny, m.dx, m.dy, w2, Ap0,
+                                              1.0, -p->focal_um / zR));
+        else
+            SHG_HIP(launch_pump_init<float2>(h->stream, static_cast<float2*>(m.P), m.nx, m.ny, m.dx, m.dy, w2, Ap0,
+                                             1.0, -p->focal_um / zR));
     }
-    SHG_HIP(hipMemsetAsync(m.S, 0, sizeof(float2) * m.N2, h->stream));
-    int rc = sum_sq(h, m.P, (long)m.N2, m.slots + 0);
-    return rc;
+    SHG_HIP(hipMemsetAsync(m.S, 0, arith_field_bytes(m.arith) * m.N2, h->stream));
+    return field_sum_sq(h, m.arith, m.P, (long)m.N2, m.slots + 0);
 }
 
 // Copies the traced state of slice g (written by row pass g) to the host
 // as FP64 and hands it to the trace callback.
 static int deliver_slice(shg_handle* h, MixedHost& m, int g) {
-    std::vector<float2> buf(2 * m.N2);
-    SHG_HIP(hipMemcpyAsync(buf.data(), m.trP, sizeof(float2) * m.N2, hipMemcpyDeviceToHost, h->stream));
-    SHG_HIP(hipMemcpyAsync(buf.data() + m.N2, m.trS, sizeof(float2) * m.N2, hipMemcpyDeviceToHost, h->stream));
-    SHG_HIP(hipStreamSynchronize(h->stream));
     SliceTrace& t = *m.trace;
     t.pump.resize(2 * m.N2);
     t.signal.resize(2 * m.N2);
-    for (size_t i = 0; i < m.N2; i++) {
-        t.pump[2 * i] = buf[i].x;
-        t.pump[2 * i + 1] = buf[i].y;
-        t.signal[2 * i] = buf[m.N2 + i].x;
-        t.signal[2 * i + 1] = buf[m.N2 + i].y;
-    }
+    if (int rc = field_to_host(h, m.arith, m.trP, m.N2, t.pump.data())) return rc;
+    if (int rc = field_to_host(h, m.arith, m.trS, m.N2, t.signal.data())) return rc;
     return t.deliver(g, m.pass);
 }
 
@@ -211,23 +216,23 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
     struct Group {
         int key_n;
         bool thermal;
-        bool fp64;
+        Arith arith;
         std::vector<MixedHost*> m;
         MixedInst* d = nullptr;
         const double2* tw = nullptr;
     };
-    std::map<std::tuple<int, bool, bool>, Group> rowg;
-    std::map<std::pair<int, bool>, Group> colg;
+    std::map<std::tuple<int, bool, int>, Group> rowg;
+    std::map<std::pair<int, int>, Group> colg;
     for (auto* m : act) {
-        auto& rg = rowg[{m->nx, m->thermal, m->fp64}];
+        auto& rg = rowg[{m->nx, m->thermal, (int)m->arith}];
         rg.key_n = m->nx;
         rg.thermal = m->thermal;
-        rg.fp64 = m->fp64;
+        rg.arith = m->arith;
         rg.m.push_back(m);
-        auto& cg = colg[{m->ny, m->fp64}];
+        auto& cg = colg[{m->ny, (int)m->arith}];
         cg.key_n = m->ny;
         cg.thermal = false;
-        cg.fp64 = m->fp64;
+        cg.arith = m->arith;
         cg.m.push_back(m);
     }
     auto by_nz = [](const MixedHost* a, const MixedHost* b) { return a->nz > b->nz; };
@@ -267,10 +272,10 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
                 maxny = std::max(maxny, m->ny);
             }
             if (!cnt) continue;
-            const int rows = plan_rows(h, gr.key_n, gr.fp64);
+            const int rows = plan_rows(h, gr.key_n, gr.arith);
             dim3 grid((maxny + rows - 1) / rows, 1, cnt);
-            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.fp64));
-            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.fp64, rows, grid, h->stream, gr.d, g, gr.tw));
+            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith));
+            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, h->stream, gr.d, g, gr.tw));
         }
         for (auto* m : act)
             if (m->trace && m->trace->wants(g, m->nz))
@@ -285,10 +290,10 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
                 maxnx = std::max(maxnx, m->nx);
             }
             if (!cnt) continue;
-            const int cols = plan_cols(h, gr.key_n, gr.fp64);
+            const int cols = plan_cols(h, gr.key_n, gr.arith);
             dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
-            ProfScope ps(h, colpass_name(gr.key_n, gr.fp64));
-            SHG_HIP(launch_colpass(gr.key_n, gr.fp64, cols, grid, h->stream, gr.d, g, gr.tw));
+            ProfScope ps(h, colpass_name(gr.key_n, gr.arith));
+            SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, h->stream, gr.d, g, gr.tw));
         }
     }
     // descriptors' host copies must outlive the async copies
@@ -297,13 +302,14 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
     return SHG_OK;
 }
 
-static size_t mixed_bytes(const MixedHost& m) {
+size_t mixed_bytes(const MixedHost& m) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t b = 4 * al(sizeof(float2) * m.N2) + 2 * al(sizeof(double2) * m.nx) +
+    const size_t fb = arith_field_bytes(m.arith);
+    size_t b = 4 * al(fb * m.N2) + 2 * al(sizeof(double2) * m.nx) +
                2 * al(sizeof(double2) * m.ny) + al(sizeof(double2) * (size_t)m.nz * 6) + al(sizeof(double) * 4) +
                2 * al(sizeof(MixedInst));
     if (m.thermal) b += heat_bytes(m.p);
-    if (m.trace) b += 2 * al(sizeof(float2) * m.N2);
+    if (m.trace) b += 2 * al(fb * m.N2);
     return b;
 }
 
@@ -343,25 +349,26 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         m.dy = m.p->ly_um / (m.ny - 1);
         m.dz = m.p->lz_um / (m.nz - 1);
         m.thermal = m.p->thermal != 0;
-        m.fp64 = m.p->precision == SHG_PREC_FP32_STORAGE;
+        m.arith = arith_of(m.p->precision);
         m.Tc = m.p->T_C;
         total += mixed_bytes(m);
     }
     total += 2 * sizeof(MixedInst) * (size_t)n + 4096;
     if (int rc = h->arena.reserve(total)) return rc;
     for (auto& m : ms) {
-        m.P = h->arena.carve<float2>(m.N2);
-        m.S = h->arena.carve<float2>(m.N2);
-        m.outP = h->arena.carve<float2>(m.N2);
-        m.outS = h->arena.carve<float2>(m.N2);
+        const size_t fb = arith_field_bytes(m.arith);
+        m.P = h->arena.carve<char>(fb * m.N2);
+        m.S = h->arena.carve<char>(fb * m.N2);
+        m.outP = h->arena.carve<char>(fb * m.N2);
+        m.outS = h->arena.carve<char>(fb * m.N2);
         m.exP = h->arena.carve<double2>(m.nx);
         m.exS = h->arena.carve<double2>(m.nx);
         m.eyP = h->arena.carve<double2>(m.ny);
         m.eyS = h->arena.carve<double2>(m.ny);
         m.phase = h->arena.carve<double2>((size_t)m.nz * 6);
         if (m.trace) {
-            m.trP = h->arena.carve<float2>(m.N2);
-            m.trS = h->arena.carve<float2>(m.N2);
+            m.trP = h->arena.carve<char>(fb * m.N2);
+            m.trS = h->arena.carve<char>(fb * m.N2);
         }
         m.slots = h->arena.carve<double>(4);
         if (!m.slots) {
@@ -448,8 +455,8 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
         }
         // the saved output state of this pass is kept in outP/outS; reduce now
         for (auto* m : act) {
-            if (int rc = sum_sq(h, m->outP, (long)m->N2, m->slots + 1)) return rc;
-            if (int rc = sum_sq(h, m->outS, (long)m->N2, m->slots + 2)) return rc;
+            if (int rc = field_sum_sq(h, m->arith, m->outP, (long)m->N2, m->slots + 1)) return rc;
+            if (int rc = field_sum_sq(h, m->arith, m->outS, (long)m->N2, m->slots + 2)) return rc;
         }
         act.clear();
         if (th.empty()) break;
@@ -484,23 +491,10 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
             r->status = SHG_ENAN;
             continue;
         }
-        if (r->pump_out || r->signal_out) {
-            std::vector<float2> buf(m.N2);
-            if (r->pump_out) {
-                SHG_HIP(hipMemcpy(buf.data(), m.outP, sizeof(float2) * m.N2, hipMemcpyDeviceToHost));
-                for (size_t i = 0; i < m.N2; i++) {
-                    r->pump_out[2 * i] = buf[i].x;
-                    r->pump_out[2 * i + 1] = buf[i].y;
-                }
-            }
-            if (r->signal_out) {
-                SHG_HIP(hipMemcpy(buf.data(), m.outS, sizeof(float2) * m.N2, hipMemcpyDeviceToHost));
-                for (size_t i = 0; i < m.N2; i++) {
-                    r->signal_out[2 * i] = buf[i].x;
-                    r->signal_out[2 * i + 1] = buf[i].y;
-                }
-            }
-        }
+        if (r->pump_out)
+            if (int rc = field_to_host(h, m.arith, m.outP, m.N2, r->pump_out)) return rc;
+        if (r->signal_out)
+            if (int rc = field_to_host(h, m.arith, m.outS, m.N2, r->signal_out)) return rc;
         if (m.thermal) {
             if (int rc = heat_finish(h, p, m.heat, r)) return rc;
         }

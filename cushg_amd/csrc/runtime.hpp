@@ -81,10 +81,10 @@ struct shg_handle {
     double* d_scalars = nullptr;  // reduction slots
     double* d_partials = nullptr;
     int n_partials = 0;
-    // tuned launch plans: rows per workgroup (row pass, by (nx, FP64
-    // arithmetic)), columns per workgroup (column pass, by (ny, FP64
-    // arithmetic)); absent = built-in default
-    std::map<std::pair<int, bool>, int> rows_plan, cols_plan;
+    // tuned launch plans: rows per workgroup (row pass, by (nx, form)),
+    // columns per workgroup (column pass, by (ny, form)); absent = built-in
+    // default
+    std::map<std::pair<int, int>, int> rows_plan, cols_plan;
     // profiling
     bool profiling = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;

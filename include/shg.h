@@ -73,8 +73,10 @@ enum { SHG_SCHEME_REFERENCE = 0, SHG_SCHEME_STRANG = 1 };
  *   FP32_STORAGE: the fused, batched march of FP32_MIXED with the same FP32
  *                 fields in device memory (same HBM bytes), but every
  *                 operation between a load and a store -- FFTs, propagator,
- *                 phase, RK4 -- in FP64 registers. */
-enum { SHG_PREC_FP32_MIXED = 0, SHG_PREC_FP64_PLAIN = 1, SHG_PREC_FP32_STORAGE = 2 };
+ *                 phase, RK4 -- in FP64 registers.
+ *   FP64_FUSED:   the fused, batched march with FP64 fields and FP64
+ *                 arithmetic (twice the HBM bytes of the FP32-field forms). */
+enum { SHG_PREC_FP32_MIXED = 0, SHG_PREC_FP64_PLAIN = 1, SHG_PREC_FP32_STORAGE = 2, SHG_PREC_FP64_FUSED = 3 };
 
 /* Heat solver for THERMAL runs.  JACOBI_REFERENCE: the reference's loop
  * (src/headers/Solver.h:332-356): Jacobi sweeps, convergence tested every
@@ -257,7 +259,7 @@ SHG_API int shg_last_kernel_times(shg_handle* h, const char** names, double* ms,
 SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
 
 /* Launch plan of the fused passes for transform length n and precision
- * (SHG_PREC_FP32_MIXED or SHG_PREC_FP32_STORAGE): rows per workgroup of
+ * (SHG_PREC_FP32_MIXED, _FP32_STORAGE or _FP64_FUSED): rows per workgroup of
  * the row pass and columns per workgroup of the column pass (0 keeps the
  * current choice).  Used by the autotuner. */
 SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t rows_per_wg, int32_t cols_per_wg);

@@ -1,10 +1,11 @@
 """GPU parity: the HIP library (through the C ABI) against the FP64 oracle,
-for both fused FP32-field forms.
+for the three fused forms.
 
 Tolerances: eta relative error <= 1e-5 against FP64 (the accuracy bar of
-the FP32 batched form).  The fields are stored in FP32: relative L2 error
-<= 2e-5 with FP32 arithmetic (SHG_PREC_FP32_MIXED), <= 2e-6 with FP64
-arithmetic (SHG_PREC_FP32_STORAGE, where only the stores round).
+the FP32 batched form) for the FP32-field forms; their fields are stored in
+FP32: relative L2 error <= 2e-5 with FP32 arithmetic (SHG_PREC_FP32_MIXED),
+<= 2e-6 with FP64 arithmetic (SHG_PREC_FP32_STORAGE, where only the stores
+round).  The FP64-field form (SHG_PREC_FP64_FUSED): 1e-10 for both.
 """
 import numpy as np
 import pytest
@@ -25,7 +26,12 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-FORMS = [pytest.param(abi.PREC_FP32_MIXED, id="mixed"), pytest.param(abi.PREC_FP32_STORAGE, id="storage")]
+FORMS = [pytest.param(abi.PREC_FP32_MIXED, id="mixed"), pytest.param(abi.PREC_FP32_STORAGE, id="storage"),
+         pytest.param(abi.PREC_FP64_FUSED, id="fp64fused")]
+# (eta, field) relative tolerances per form: the FP64-field form differs
+# from the oracle only in FP64 rounding of a different operation order
+TOLS = {abi.PREC_FP32_MIXED: (ETA_TOL, FIELD_TOL), abi.PREC_FP32_STORAGE: (ETA_TOL, 2e-6),
+        abi.PREC_FP64_FUSED: (1e-10, 1e-10)}
 
 
 @pytest.fixture(params=FORMS)
@@ -33,11 +39,10 @@ def form(request):
     return request.param
 
 
-def _check(p, handle, form=abi.PREC_FP32_MIXED, eta_tol=ETA_TOL, field_tol=None):
+def _check(p, handle, form=abi.PREC_FP32_MIXED):
     p = p.copy()
     p.precision = form
-    if field_tol is None:
-        field_tol = FIELD_TOL if form == abi.PREC_FP32_MIXED else 2e-6
+    eta_tol, field_tol = TOLS[form]
     print(f"case nx={p.nx} ny={p.ny} nz={p.nz} scheme={p.scheme} thermal={p.thermal} precision={form}",
           flush=True)
     ro, o = O.run(p)
@@ -49,8 +54,8 @@ def _check(p, handle, form=abi.PREC_FP32_MIXED, eta_tol=ETA_TOL, field_tol=None)
     e_s = rel_l2(g["signal"], o["signal"])
     print(f"  eta gpu={rg.eta:.12e} cpu={ro.eta:.12e} rel={e_eta:.2e}  pump {e_p:.2e}  sh {e_s:.2e}")
     assert e_eta <= eta_tol
-    assert rel(rg.P_pump_in_W, ro.P_pump_in_W) <= 1e-6
-    assert rel(rg.P_pump_out_W, ro.P_pump_out_W) <= 1e-6
+    assert rel(rg.P_pump_in_W, ro.P_pump_in_W) <= min(1e-6, 10 * eta_tol)
+    assert rel(rg.P_pump_out_W, ro.P_pump_out_W) <= min(1e-6, 10 * eta_tol)
     assert e_p <= field_tol
     assert e_s <= field_tol
     return ro, rg
