@@ -39,7 +39,7 @@ struct AdjRun {
     Arith arith;
     bool f64;
     size_t fb;                   // bytes per complex field value
-    MixedInst* d = nullptr;      // device descriptor
+    MixedInst* d = nullptr;      // device descriptors: [0] forward, [1 + k] recomputation of segment k
     const double2* twx = nullptr;
     const double2* twy = nullptr;
     int rows = 0, cols = 0;
@@ -48,25 +48,21 @@ struct AdjRun {
         return static_cast<char*>(base) + (slot * 2 + f) * fb * m.N2;
     }
 
-    int upload_desc() {
-        SHG_HIP(hipMemcpyAsync(d, &m.desc, sizeof(MixedInst), hipMemcpyHostToDevice, h->stream));
-        return SHG_OK;
-    }
-    int row(int g) {
+    int row(int g, const MixedInst* di, hipStream_t st) {
         dim3 grid((m.ny + rows - 1) / rows, 1, 1);
-        ProfScope ps(h, rowpass_name(m.nx, false, arith));
-        SHG_HIP(launch_rowpass(m.nx, false, arith, rows, grid, h->stream, d, g, twx));
+        ProfScope ps(h, rowpass_name(m.nx, false, arith), st);
+        SHG_HIP(launch_rowpass(m.nx, false, arith, rows, grid, st, di, g, twx));
         return SHG_OK;
     }
-    int col(int g) {
+    int col(int g, const MixedInst* di, hipStream_t st) {
         dim3 grid((m.nx + cols - 1) / cols, 2, 1);
-        ProfScope ps(h, colpass_name(m.ny, arith));
-        SHG_HIP(launch_colpass(m.ny, arith, cols, grid, h->stream, d, g, twy));
+        ProfScope ps(h, colpass_name(m.ny, arith), st);
+        SHG_HIP(launch_colpass(m.ny, arith, cols, grid, st, di, g, twy));
         return SHG_OK;
     }
-    int copy_fields(void* dstP, void* dstS, const void* srcP, const void* srcS) {
-        SHG_HIP(hipMemcpyAsync(dstP, srcP, fb * m.N2, hipMemcpyDeviceToDevice, h->stream));
-        SHG_HIP(hipMemcpyAsync(dstS, srcS, fb * m.N2, hipMemcpyDeviceToDevice, h->stream));
+    int copy_fields(void* dstP, void* dstS, const void* srcP, const void* srcS, hipStream_t st) {
+        SHG_HIP(hipMemcpyAsync(dstP, srcP, fb * m.N2, hipMemcpyDeviceToDevice, st));
+        SHG_HIP(hipMemcpyAsync(dstS, srcS, fb * m.N2, hipMemcpyDeviceToDevice, st));
         return SHG_OK;
     }
 };
@@ -115,10 +111,14 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     A.rows = plan_rows(h, m.nx, A.arith);
     A.cols = plan_cols(h, m.ny, A.arith);
 
+    // segment k covers slices [kK, min(kK + K, nsl)); its recomputation
+    // keeps the column-pass output C_s of each of its slices in one of two
+    // history buffers (segment k uses buffer k % 2), so that segment k - 1
+    // is recomputed on a second stream while the adjoint of segment k runs
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t two = 2 * A.fb * m.N2;
-    size_t total = mixed_bytes(m) + al(sizeof(MixedInst)) + al(two * ncp) + al(two * (K + 1)) + al(two) +
-                   al(sizeof(double) * ((size_t)nz * nbr * 12 + (size_t)nz * nbc * 2 + (size_t)nbp * 3)) +
+    size_t total = mixed_bytes(m) + al(sizeof(MixedInst) * (ncp + 1)) + al(two * ncp) + 2 * al(two * K) +
+                   al(two) + al(sizeof(double) * ((size_t)nz * nbr * 12 + (size_t)nz * nbc * 2 + (size_t)nbp * 3)) +
                    al(sizeof(double) * ((size_t)nz * 12 + (size_t)nz * 2 + 3)) + 8192;
     if (int rc = h->arena.reserve(total)) return rc;
     Arena& ar = h->arena;
@@ -132,9 +132,9 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     m.eyS = ar.carve<double2>(m.ny);
     m.phase = ar.carve<double2>((size_t)nz * 6);
     m.slots = ar.carve<double>(4);
-    A.d = ar.carve<MixedInst>(1);
+    A.d = ar.carve<MixedInst>((size_t)ncp + 1);
     void* cp = ar.carve<char>(two * ncp);
-    void* hist = ar.carve<char>(two * (K + 1));
+    void* hist[2] = {ar.carve<char>(two * K), ar.carve<char>(two * K)};
     void* lam = ar.carve<char>(two);
     double* partR = ar.carve<double>((size_t)nz * nbr * 12);
     double* partC = ar.carve<double>((size_t)nz * nbc * 2);
@@ -142,7 +142,7 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     double* G = ar.carve<double>((size_t)nz * 12);
     double* Gpsi = ar.carve<double>((size_t)nz * 2);
     double* Gpump = ar.carve<double>(3);
-    if (!Gpump || !A.d || !cp || !hist || !lam) {
+    if (!Gpump || !A.d || !cp || !hist[0] || !hist[1] || !lam) {
         set_error("arena exhausted (adjoint)");
         return SHG_ENOMEM;
     }
@@ -164,47 +164,49 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     dsc.phase = m.phase;
     dsc.trace_every = 1;
     if (int rc = setup_fields(h, m)) return rc;  // tables (into desc), pump, zero SH, P_in
-    if (int rc = A.upload_desc()) return rc;
+    // descriptors: the forward records the last segment's history itself;
+    // descriptor 1 + k recomputes segment k
+    auto seg = [&](int k) { return std::make_pair(k * K, std::min(k * K + K, nsl)); };
+    std::vector<MixedInst> descs((size_t)ncp + 1, dsc);
+    for (int k = -1; k < ncp; k++) {
+        const int kk = k < 0 ? ncp - 1 : k;
+        MixedInst& q = descs[(size_t)(k + 1)];
+        q.hist = hist[kk % 2];
+        q.hist_g0 = seg(kk).first;
+        q.hist_n = seg(kk).second - seg(kk).first;
+    }
+    SHG_HIP(hipMemcpyAsync(A.d, descs.data(), sizeof(MixedInst) * descs.size(), hipMemcpyHostToDevice, h->stream));
     SHG_HIP(hipMemsetAsync(partR, 0, sizeof(double) * (size_t)nz * nbr * 12, h->stream));
     SHG_HIP(hipMemsetAsync(partC, 0, sizeof(double) * (size_t)nz * nbc * 2, h->stream));
+    hipStream_t s1 = h->stream, s2 = h->stream2;
+    hipEvent_t ev_fwd = h->ev[0], ev_rec[2] = {h->ev[1], h->ev[2]}, ev_adj[2] = {h->ev[3], h->ev[4]};
 
     // 1. forward march with checkpoints (state before slices 0, K, 2K, ...)
-    if (int rc = A.copy_fields(A.field(cp, 0, 0), A.field(cp, 0, 1), m.P, m.S)) return rc;
     for (int g = 0; g < nz; g++) {
-        if (int rc = A.row(g)) return rc;  // row pass nz-1 saves the output state
+        if (g % K == 0 && g / K < ncp)
+            if (int rc = A.copy_fields(A.field(cp, g / K, 0), A.field(cp, g / K, 1), m.P, m.S, s1)) return rc;
+        if (int rc = A.row(g, A.d, s1)) return rc;  // row pass nz-1 saves the output state
         if (g == nz - 1) break;
-        if (int rc = A.col(g)) return rc;
-        const int nx = g + 1;
-        if (nx % K == 0 && nx / K < ncp)
-            if (int rc = A.copy_fields(A.field(cp, nx / K, 0), A.field(cp, nx / K, 1), m.P, m.S)) return rc;
+        if (int rc = A.col(g, A.d, s1)) return rc;
     }
     if (int rc = field_sum_sq(h, A.arith, m.outP, (long)m.N2, m.slots + 1)) return rc;
     if (int rc = field_sum_sq(h, A.arith, m.outS, (long)m.N2, m.slots + 2)) return rc;
     double sl[3];
-    SHG_HIP(hipMemcpyAsync(sl, m.slots, sizeof sl, hipMemcpyDeviceToHost, h->stream));
-    SHG_HIP(hipStreamSynchronize(h->stream));
+    SHG_HIP(hipMemcpyAsync(sl, m.slots, sizeof sl, hipMemcpyDeviceToHost, s1));
+    SHG_HIP(hipEventRecord(ev_fwd, s1));
+    SHG_HIP(hipStreamWaitEvent(s2, ev_fwd, 0));
     const double lp = p->lambda_p_um, ls = 0.5 * lp, Tc = p->T_C;
     const double np = refr_index<double>(p->crystal, lp, Tc), ns = refr_index<double>(p->crystal, ls, Tc);
     const double cpow = 0.5 * kEps0 * kC * m.dx * m.dy;
-    res->P_pump_in_W = cpow * np * sl[0];
-    res->P_pump_out_W = cpow * np * sl[1];
-    res->P_sh_W = cpow * ns * sl[2];
-    res->eta = p->power_W > 0 ? res->P_sh_W / p->power_W : 0.0;
-    res->steps_done = nz;
-    res->outer_iters = 1;
-    res->status = SHG_OK;
-    if (!std::isfinite(res->eta)) {
-        set_error("non-finite field power");
-        return SHG_ENAN;
-    }
 
     // 2. seed: d eta / d S_out = 2 c n_s dx dy / P * S_out
     void* lamP = A.field(lam, 0, 0);
     void* lamS = A.field(lam, 0, 1);
     const double coef = p->power_W > 0 ? 2.0 * cpow * ns / p->power_W : 0.0;
-    SHG_HIP(launch_adj_seed(A.f64, lamP, lamS, m.outS, (long)m.N2, coef, h->stream));
+    SHG_HIP(launch_adj_seed(A.f64, lamP, lamS, m.outS, (long)m.N2, coef, s1));
 
-    // 3. segments backwards
+    // 3. segments backwards; the recomputation of segment k - 1 (stream 2)
+    // overlaps the adjoint of segment k (stream 1)
     AdjArgs aa{};
     aa.I = A.d;
     aa.lamP = lamP;
@@ -212,55 +214,56 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     aa.dfx = 1.0 / (m.dx * m.nx);
     aa.dfy = 1.0 / (m.dy * m.ny);
     for (int k = ncp - 1; k >= 0; k--) {
-        const int a = k * K, b = std::min(a + K, nsl);
-        // slot 0: the state before slice a; slots 1..: column-pass outputs of slices a..b-1
-        if (int rc = A.copy_fields(m.P, m.S, A.field(cp, k, 0), A.field(cp, k, 1))) return rc;
-        if (int rc = A.copy_fields(A.field(hist, 0, 0), A.field(hist, 0, 1), m.P, m.S)) return rc;
-        dsc.hist = A.field(hist, 1, 0);
-        dsc.hist_g0 = a;
-        dsc.hist_n = b - a;
-        if (int rc = A.upload_desc()) return rc;
-        for (int g = a; g < b; g++) {
-            if (int rc = A.row(g)) return rc;
-            if (int rc = A.col(g)) return rc;
+        if (k >= 1) {
+            const int j = k - 1, a = seg(j).first, b = seg(j).second;
+            // history buffer j % 2 was last read by the adjoint of segment j + 2
+            if (j + 2 <= ncp - 1) SHG_HIP(hipStreamWaitEvent(s2, ev_adj[j % 2], 0));
+            if (int rc = A.copy_fields(m.P, m.S, A.field(cp, j, 0), A.field(cp, j, 1), s2)) return rc;
+            for (int g = a; g < b; g++) {
+                if (int rc = A.row(g, A.d + 1 + j, s2)) return rc;
+                if (int rc = A.col(g, A.d + 1 + j, s2)) return rc;
+            }
+            SHG_HIP(hipEventRecord(ev_rec[j % 2], s2));
         }
-        dsc.hist = nullptr;
-        dsc.hist_n = 0;
-        if (int rc = A.upload_desc()) return rc;
+        if (k != ncp - 1) SHG_HIP(hipStreamWaitEvent(s1, ev_rec[k % 2], 0));
+        const int a = seg(k).first, b = seg(k).second;
+        void* H = hist[k % 2];
+        // C_s of slice s; C_{a-1} (or the initial spatial state at a = 0) is the checkpoint
+        auto fwd = [&](int s, int f) { return s < a ? A.field(cp, k, f) : A.field(H, (size_t)(s - a), f); };
         for (int s = b - 1; s >= a; s--) {
-            const size_t hs = (size_t)(s - a + 1);  // history slot of C_s
             if (s == nsl - 1) {
                 // adjoint of R2_{nz-2} from the seed (spatial): boundary nz-1, second half only
                 aa.s = nz - 1;
                 aa.first = 0;
                 aa.second = 1;
                 aa.fwd_spatial = 0;
-                aa.fwdP = A.field(hist, hs, 0);
-                aa.fwdS = A.field(hist, hs, 1);
+                aa.fwdP = fwd(s, 0);
+                aa.fwdS = fwd(s, 1);
                 aa.part = partR + (size_t)(nz - 1) * nbr * 12;
                 ProfScope ps(h, "adjoint_rowpass");
-                SHG_HIP(launch_adj_row(A.f64, m.nx, m.ny, h->stream, aa, A.twx));
+                SHG_HIP(launch_adj_row(A.f64, m.nx, m.ny, s1, aa, A.twx));
             }
             {
                 aa.s = s;
-                aa.fwdP = A.field(hist, hs, 0);
-                aa.fwdS = A.field(hist, hs, 1);
+                aa.fwdP = fwd(s, 0);
+                aa.fwdS = fwd(s, 1);
                 aa.part = partC + (size_t)s * nbc * 2;
                 ProfScope ps(h, "adjoint_colpass");
-                SHG_HIP(launch_adj_col(A.f64, m.nx, m.ny, h->stream, aa, A.twy));
+                SHG_HIP(launch_adj_col(A.f64, m.nx, m.ny, s1, aa, A.twy));
             }
             {
                 aa.s = s;
                 aa.first = 1;
                 aa.second = s >= 1;
                 aa.fwd_spatial = s == 0;
-                aa.fwdP = A.field(hist, hs - 1, 0);  // C_{s-1}, or X_0 at s = 0
-                aa.fwdS = A.field(hist, hs - 1, 1);
+                aa.fwdP = fwd(s - 1, 0);  // C_{s-1}, or X_0 at s = 0
+                aa.fwdS = fwd(s - 1, 1);
                 aa.part = partR + (size_t)s * nbr * 12;
                 ProfScope ps(h, "adjoint_rowpass");
-                SHG_HIP(launch_adj_row(A.f64, m.nx, m.ny, h->stream, aa, A.twx));
+                SHG_HIP(launch_adj_row(A.f64, m.nx, m.ny, s1, aa, A.twx));
             }
         }
+        SHG_HIP(hipEventRecord(ev_adj[k % 2], s1));
     }
     // 4. initial-field gradients and the per-slice sums
     const double w2 = p->waist_um * p->waist_um;
@@ -276,6 +279,17 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     SHG_HIP(hipMemcpyAsync(gpsi.data(), Gpsi, sizeof(double) * gpsi.size(), hipMemcpyDeviceToHost, h->stream));
     SHG_HIP(hipMemcpyAsync(gp.data(), Gpump, sizeof(double) * 3, hipMemcpyDeviceToHost, h->stream));
     SHG_HIP(hipStreamSynchronize(h->stream));
+    res->P_pump_in_W = cpow * np * sl[0];
+    res->P_pump_out_W = cpow * np * sl[1];
+    res->P_sh_W = cpow * ns * sl[2];
+    res->eta = p->power_W > 0 ? res->P_sh_W / p->power_W : 0.0;
+    res->steps_done = nz;
+    res->outer_iters = 1;
+    res->status = SHG_OK;
+    if (!std::isfinite(res->eta)) {
+        set_error("non-finite field power");
+        return SHG_ENAN;
+    }
 
     // assembly: slice s gets R1_s from row-pass launch s ([0..5]) and R2_s
     // from launch s+1 ([6..11])

@@ -185,9 +185,14 @@ int shg_handle_create(int32_t device, shg_handle** out) {
     shg_handle* h = new shg_handle();
     h->device = device;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
+    for (int i = 0; i < 6 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&h->ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->d_scalars), sizeof(double) * 64);
     if (e != hipSuccess) {
         set_error("handle create: %s", hipGetErrorString(e));
+        for (auto ev : h->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (h->stream2) (void)hipStreamDestroy(h->stream2);
         if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return SHG_EDEVICE;
@@ -208,6 +213,12 @@ int shg_handle_destroy(shg_handle* h) {
     if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
     heat_cache_free(h);
+    for (auto ev : h->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (h->stream2) {
+        (void)hipStreamSynchronize(h->stream2);
+        (void)hipStreamDestroy(h->stream2);
+    }
     (void)hipStreamDestroy(h->stream);
     delete h;
     return SHG_OK;

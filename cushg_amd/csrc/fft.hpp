@@ -158,17 +158,22 @@ template <typename C, bool INV, int R> __device__ __forceinline__ void dftR(C* x
 // Every thread of the workgroup must call it the same number of times
 // (it contains workgroup barriers).
 // NF independent transforms (e.g. the pump and SH rows) sharing the same
-// barriers: v[f] is held by thread tp, buffer f at lds + f*ld.
+// barriers: v[f] is held by thread tp, buffer f at lds + f*ld.  A thread
+// with act = false only takes part in the barriers (it neither reads nor
+// writes `lds` and leaves v unchanged).
 template <typename C, int N, bool INV, int NF>
-__device__ __forceinline__ void fft_multi(C (&v)[NF][8], C* lds, int ld, const tw_t<C>* __restrict__ tw, int tp) {
+__device__ __forceinline__ void fft_multi(C (&v)[NF][8], C* lds, int ld, const tw_t<C>* __restrict__ tw, int tp,
+                                          bool act = true) {
     constexpr int TPR = FftShape<N>::TPR;
     constexpr int M = FftShape<N>::M;
     // pass 1: span 1, radix 8, butterfly t = tp reads x[tp + j*TPR] = v[j]
+    if (act) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-        dft8<C, INV>(v[f]);
+        for (int f = 0; f < NF; f++) {
+            dft8<C, INV>(v[f]);
 #pragma unroll
-        for (int q = 0; q < 8; q++) lds[f * ld + fpad(8 * tp + q)] = v[f][q];
+            for (int q = 0; q < 8; q++) lds[f * ld + fpad(8 * tp + q)] = v[f][q];
+        }
     }
     __syncthreads();
     if constexpr (M > 1) {
@@ -180,7 +185,7 @@ __device__ __forceinline__ void fft_multi(C (&v)[NF][8], C* lds, int ld, const t
 #pragma unroll
             for (int i = 0; i < PER; i++) {
                 const int b = tp + i * TPR;
-                if (PER * TPR == 64 || b < 64) {
+                if (act && (PER * TPR == 64 || b < 64)) {
                     const int k = b & 7;
 #pragma unroll
                     for (int j = 0; j < M; j++) {
@@ -196,7 +201,7 @@ __device__ __forceinline__ void fft_multi(C (&v)[NF][8], C* lds, int ld, const t
 #pragma unroll
             for (int i = 0; i < PER; i++) {
                 const int b = tp + i * TPR;
-                if (PER * TPR == 64 || b < 64) {
+                if (act && (PER * TPR == 64 || b < 64)) {
                     const int k = b & 7;
                     dftR<C, INV, M>(xs[f][i]);
                     const int base = (b - k) * M + k;
@@ -207,22 +212,25 @@ __device__ __forceinline__ void fft_multi(C (&v)[NF][8], C* lds, int ld, const t
         __syncthreads();
     }
     // pass 3: span N/8, radix 8, butterfly t = tp, twiddle W_N^{j tp}
+    if (act) {
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
+        for (int f = 0; f < NF; f++) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            C x = lds[f * ld + fpad(tp + j * TPR)];
-            if (j > 0) x = twmul<INV>(x, tw, j * tp);
-            v[f][j] = x;
+            for (int j = 0; j < 8; j++) {
+                C x = lds[f * ld + fpad(tp + j * TPR)];
+                if (j > 0) x = twmul<INV>(x, tw, j * tp);
+                v[f][j] = x;
+            }
+            dft8<C, INV>(v[f]);
         }
-        dft8<C, INV>(v[f]);
     }
     __syncthreads();  // the buffers may be reused by the next call
 }
 
 template <typename C, int N, bool INV>
-__device__ __forceinline__ void fft_dist(C (&v)[8], C* lds, const tw_t<C>* __restrict__ tw, int tp) {
-    fft_multi<C, N, INV, 1>(reinterpret_cast<C(&)[1][8]>(v), lds, 0, tw, tp);
+__device__ __forceinline__ void fft_dist(C (&v)[8], C* lds, const tw_t<C>* __restrict__ tw, int tp,
+                                         bool act = true) {
+    fft_multi<C, N, INV, 1>(reinterpret_cast<C(&)[1][8]>(v), lds, 0, tw, tp, act);
 }
 
 }  // namespace shg

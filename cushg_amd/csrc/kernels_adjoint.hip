@@ -125,42 +125,51 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], double* part, double
     }
 }
 
-// Adjoint row pass at boundary s: rows of ROWS per workgroup, NX/8 threads
-// per row; grid (ny/ROWS).  Partial sums per workgroup (12):
+// Adjoint row pass at boundary s: ROWS rows per workgroup of NX * ROWS
+// threads, grid (ny / ROWS).  The transforms run on groups of NX/8 threads
+// holding 8 points each (group g: row g / 4, field g % 4 = lam_P, lam_S,
+// Y_P, Y_S; the groups beyond 4 ROWS only join the barriers); the pointwise
+// reverse RK4 work, which dominates, runs one point per thread from LDS so
+// that a single 256 x 256 instance still spreads over every SIMD.
+// Partial sums per workgroup (12):
 //   [0..5]  call R1_s:     dL/dphi_0..2, dL/dkp, dL/dks, dL/dbfac
 //   [6..11] call R2_{s-1}: the same
 template <int NX, int ROWS, typename S>
-__global__ __launch_bounds__(NX / 8 * ROWS) void k_adj_row(AdjArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NX * ROWS) void k_adj_row(AdjArgs a, const double2* __restrict__ tw) {
     using C = double2;
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
-    __shared__ C lds[ROWS * LD];
+    __shared__ C lds[4 * ROWS * LD];
     __shared__ double red[16 * 12];
     const MixedInst& I = *a.I;
-    const int y = blockIdx.x * ROWS + threadIdx.y;
-    const int tp = threadIdx.x;
-    C* my = lds + threadIdx.y * LD;
-    const size_t base = (size_t)y * NX + tp;
-    C lf[2][8], yf[2][8];
+    const int t = threadIdx.x;
+    const int gi = t / TPR, tp = t % TPR;
     const S* fP = static_cast<const S*>(a.fwdP);
     const S* fS = static_cast<const S*>(a.fwdS);
     S* LP = static_cast<S*>(a.lamP);
     S* LS = static_cast<S*>(a.lamS);
+    const size_t row0 = (size_t)blockIdx.x * ROWS * NX;
+    // 1. the four rows per image row into LDS, spatial
+    {
+        const bool act = gi < 4 * ROWS;
+        const int f = gi & 3;
+        const S* src = f == 0 ? LP : f == 1 ? LS : f == 2 ? fP : fS;
+        const size_t base = row0 + (size_t)(gi >> 2) * NX + tp;
+        C v[8];
+        if (act) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        lf[0][q] = from_store<C>(LP[base + q * TPR]);
-        lf[1][q] = from_store<C>(LS[base + q * TPR]);
-        yf[0][q] = from_store<C>(fP[base + q * TPR]);
-        yf[1][q] = from_store<C>(fS[base + q * TPR]);
+            for (int q = 0; q < 8; q++) v[q] = from_store<C>(src[base + q * TPR]);
+        }
+        C* buf = lds + (act ? gi : 0) * LD;
+        if (a.first || !a.fwd_spatial)
+            fft_dist<C, NX, true>(v, buf, tw, tp, act && (f < 2 ? a.first != 0 : !a.fwd_spatial));
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) buf[fpad(tp + q * TPR)] = v[q];
+        }
     }
-    if (a.first) {
-        fft_dist<C, NX, true>(lf[0], my, tw, tp);
-        fft_dist<C, NX, true>(lf[1], my, tw, tp);
-    }
-    if (!a.fwd_spatial) {
-        fft_dist<C, NX, true>(yf[0], my, tw, tp);
-        fft_dist<C, NX, true>(yf[1], my, tw, tp);
-    }
+    __syncthreads();
+    // 2. one point per thread
     const RkConst<double> k{I.c1, I.c2, I.c3, I.w, I.ap_h, I.as_h};
     const int s = a.s;
     C e1[3], e2[3];
@@ -174,27 +183,43 @@ __global__ __launch_bounds__(NX / 8 * ROWS) void k_adj_row(AdjArgs a, const doub
         }
     }
     double acc1[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const C Yp = yf[0][q], Ys = yf[1][q];
+    {
+        C* rb = lds + (t / NX) * 4 * LD;
+        const int xi = fpad(t % NX);
+        const C Yp = rb[2 * LD + xi], Ys = rb[3 * LD + xi];
         C Xp = Yp, Xs = Ys;
         if (!a.fwd_spatial) rk4_point(Xp, Xs, e2, I.kp, I.ks, I.bfac, k);  // X_s = R2_{s-1}(Y_{s-1})
-        C lp = lf[0][q], ls = lf[1][q];
+        C lp = rb[xi], ls = rb[LD + xi];
         if (a.first) rk4_adj(lp, ls, Xp, Xs, e1, I.kp, I.ks, I.bfac, k, acc1);
         if (a.second) rk4_adj(lp, ls, Yp, Ys, e2, I.kp, I.ks, I.bfac, k, acc2);
-        lf[0][q] = lp;
-        lf[1][q] = ls;
+        rb[xi] = lp;
+        rb[LD + xi] = ls;
     }
-    if (a.second) {
-        fft_dist<C, NX, false>(lf[0], my, tw, tp);
-        fft_dist<C, NX, false>(lf[1], my, tw, tp);
-    }
-    const uint32_t kp = store_key(4, 0, s), ks = store_key(4, 1, s);
+    __syncthreads();
+    // 3. lam_P, lam_S back to row-spectral form and out
+    {
+        const bool act = gi < 2 * ROWS;
+        const int r = gi >> 1, f = gi & 1;
+        C* buf = lds + (act ? r * 4 + f : 0) * LD;
+        C v[8];
+        if (act) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const size_t i = base + q * TPR;
-        LP[i] = to_store<S>(lf[0][q], i, kp);
-        LS[i] = to_store<S>(lf[1][q], i, ks);
+            for (int q = 0; q < 8; q++) v[q] = buf[fpad(tp + q * TPR)];
+        }
+        if (a.second) {
+            __syncthreads();  // every group has read its buffer before the transform overwrites it
+            fft_dist<C, NX, false>(v, buf, tw, tp, act);
+        }
+        if (act) {
+            S* dst = f == 0 ? LP : LS;
+            const uint32_t key = store_key(4, f, s);
+            const size_t base = row0 + (size_t)r * NX + tp;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const size_t i = base + q * TPR;
+                dst[i] = to_store<S>(v[q], i, key);
+            }
+        }
     }
     double v[12];
 #pragma unroll
@@ -347,14 +372,14 @@ __global__ __launch_bounds__(256) void k_adj_reduce(const double* __restrict__ p
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
-template <int NX> constexpr int adj_rows() { return NX / 8 >= 64 ? 1 : 64 / (NX / 8); }
+template <int NX> constexpr int adj_rows() { return NX >= 256 ? 1 : 256 / NX; }  // ~256 threads per workgroup
 
 template <typename S>
 static hipError_t adj_row_n(int nx, dim3 grid, hipStream_t st, const AdjArgs& a, const double2* tw) {
     switch (nx) {
 #define SHG_ADJROW(N)                                                                                 \
     case N:                                                                                           \
-        hipLaunchKernelGGL((k_adj_row<N, adj_rows<N>(), S>), grid, dim3(N / 8, adj_rows<N>()), 0, st, a, tw); \
+        hipLaunchKernelGGL((k_adj_row<N, adj_rows<N>(), S>), grid, dim3(N * adj_rows<N>()), 0, st, a, tw);      \
         return hipGetLastError();
         SHG_ADJROW(64)
         SHG_ADJROW(128)
@@ -385,9 +410,13 @@ hipError_t launch_adj_row(bool f64, int nx, int ny, hipStream_t st, const AdjArg
     return f64 ? adj_row_n<double2>(nx, grid, st, a, tw) : adj_row_n<float2>(nx, grid, st, a, tw);
 }
 
+// two columns per workgroup: the adjoint runs one instance, so the grid
+// (nx / 2 x 2 fields) is what spreads it over the CUs
+constexpr int kAdjCols = 2;
+
 template <typename S>
 static hipError_t adj_col_n(int ny, int nx, hipStream_t st, const AdjArgs& a, const double2* tw) {
-    constexpr int COLS = 8;
+    constexpr int COLS = kAdjCols;
     const dim3 grid(nx / COLS, 2);
     switch (ny) {
 #define SHG_ADJCOL(N)                                                                                 \
@@ -405,7 +434,7 @@ static hipError_t adj_col_n(int ny, int nx, hipStream_t st, const AdjArgs& a, co
     return hipErrorInvalidValue;
 }
 
-int adj_col_workgroups(int nx) { return nx / 8 * 2; }
+int adj_col_workgroups(int nx) { return nx / kAdjCols * 2; }
 
 hipError_t launch_adj_col(bool f64, int nx, int ny, hipStream_t st, const AdjArgs& a, const double2* tw) {
     return f64 ? adj_col_n<double2>(ny, nx, st, a, tw) : adj_col_n<float2>(ny, nx, st, a, tw);

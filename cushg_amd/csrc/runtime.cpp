@@ -44,7 +44,7 @@ int Arena::reserve(size_t bytes) {
     return SHG_OK;
 }
 
-ProfScope::ProfScope(shg_handle* h_, const char* n) : h(h_), name(n) {
+ProfScope::ProfScope(shg_handle* h_, const char* n, hipStream_t s) : h(h_), name(n), st(s ? s : h_->stream) {
     if (!h->profiling) return;
     if (h->event_pool.size() < 2) {
         for (int i = 0; i < 64; i++) {
@@ -57,12 +57,12 @@ ProfScope::ProfScope(shg_handle* h_, const char* n) : h(h_), name(n) {
     h->event_pool.pop_back();
     b = h->event_pool.back();
     h->event_pool.pop_back();
-    (void)hipEventRecord(a, h->stream);
+    (void)hipEventRecord(a, st);
 }
 
 ProfScope::~ProfScope() {
     if (!a) return;
-    (void)hipEventRecord(b, h->stream);
+    (void)hipEventRecord(b, st);
     h->pending.push_back({name, {a, b}});
     if (h->pending.size() > 4096) prof_collect(h);
 }
@@ -70,6 +70,7 @@ ProfScope::~ProfScope() {
 void prof_collect(shg_handle* h) {
     if (h->pending.empty()) return;
     (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     for (auto& pe : h->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {

@@ -74,6 +74,10 @@ struct HeatGeom;
 struct shg_handle {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for work that overlaps the main
+    // stream's (the adjoint's segment recomputation), and ordering events
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev[6] = {};
     shg::Arena arena;
     std::map<int, double2*> tw64;
     void* blas = nullptr;         // rocBLAS handle (heat solver), created lazily
@@ -100,7 +104,8 @@ struct ProfScope {
     shg_handle* h;
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(shg_handle* h_, const char* n);
+    hipStream_t st;
+    ProfScope(shg_handle* h_, const char* n, hipStream_t s = nullptr);
     ~ProfScope();
 };
 void prof_collect(shg_handle* h);

@@ -137,6 +137,9 @@ def test_adjoint256_vs_gpu_fp64_finite_differences(handle):
     ta = time.perf_counter() - t
     print(f"  forward {tf * 1e3:.1f} ms, forward + adjoint {ta * 1e3:.1f} ms, ratio {ta / tf:.2f}")
     assert abs(r.eta - r0.eta) / r0.eta < 1e-12
+    # the gradient (forward, recomputation and reverse sweep) costs at most
+    # three forwards (measured about 2.2)
+    assert ta / tf <= 3.0
     q = p.copy()
     q.precision = abi.PREC_FP64_FUSED
 
