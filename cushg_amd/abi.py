@@ -38,6 +38,8 @@ PREC_FP32_STORAGE = 2
 PREC_FP64_FUSED = 3
 HEAT_JACOBI_REFERENCE = 0
 HEAT_DIRECT = 1
+# row-pass plan value of the wide form (include/shg.h SHG_PLAN_ROWS_WIDE)
+PLAN_ROWS_WIDE = 16
 HEAT_BC_OPEN_TOP = 0
 HEAT_BC_OVEN_SURROUNDED = 1
 
@@ -60,6 +62,7 @@ class ShgParams(C.Structure):
         ("max_outer", C.c_int32), ("check_every", C.c_int32), ("heat_bc", C.c_int32),
         ("max_sweeps", C.c_int64),
         ("jacobi_tol", C.c_double), ("heat_tol_K", C.c_double),
+        ("waist_y_um", C.c_double),
         ("poling_phase", C.POINTER(C.c_double)),
     ]
 

@@ -49,7 +49,8 @@ struct AdjRun {
     }
 
     int row(int g, const MixedInst* di, hipStream_t st) {
-        dim3 grid((m.ny + rows - 1) / rows, 1, 1);
+        const int per = rowpass_rows_per_wg(m.nx, rows);
+        dim3 grid((m.ny + per - 1) / per, 1, 1);
         ProfScope ps(h, rowpass_name(m.nx, false, arith), st);
         SHG_HIP(launch_rowpass(m.nx, false, arith, rows, grid, st, di, g, twx));
         return SHG_OK;
@@ -77,6 +78,10 @@ double dn_dT(int crystal, double lam, double T) {
 int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r) {
     if (p->thermal) {
         set_error("shg_adjoint: gradients of THERMAL runs are not provided (the heat solve is not differentiated)");
+        return SHG_EUNSUPPORTED;
+    }
+    if (p->waist_y_um > 0) {
+        set_error("shg_adjoint: the gradient is taken for the circular pump (waist_y_um = 0)");
         return SHG_EUNSUPPORTED;
     }
     const auto t0 = std::chrono::steady_clock::now();
@@ -109,7 +114,7 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     const int ncp = (nsl + K - 1) / K;
     const int nbr = adj_row_workgroups(m.nx, m.ny), nbc = adj_col_workgroups(m.nx), nbp = adj_pump_workgroups();
     A.rows = plan_rows(h, m.nx, A.arith);
-    A.cols = plan_cols(h, m.ny, A.arith);
+    A.cols = plan_cols(h, m.ny, A.arith, m.nx);
 
     // segment k covers slices [kK, min(kK + K, nsl)); its recomputation
     // keeps the column-pass output C_s of each of its slices in one of two

@@ -3,10 +3,12 @@
 
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <vector>
 
 #include "common.hpp"
+#include "kernels.hpp"
 #include "runtime.hpp"
 #include <rocblas/rocblas.h>
 
@@ -17,6 +19,29 @@ int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std
 int run_plain(shg_handle* h, const shg_params* p, shg_result* r, SliceTrace* trace = nullptr);
 int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r);
 void heat_cache_free(shg_handle* h);
+
+// setPump (src/headers/Efields.h:84-119): z_R = pi n_p w^2 / lambda_p,
+// M = 1 - i f / z_R, Ap0 = sqrt(4 P / (eps0 c pi n_p w^2)), A = Ap0 / M
+// exp(-r^2 / (w^2 M)).  Elliptical variant (the commented-out form at
+// src/headers/Efields.h:113-115): per-axis M, amplitude over sqrt(Mx My),
+// and Ap0 with wx wy in place of w^2 so the beam carries power_W.
+PumpCoef pump_coef(const shg_params* p, double np) {
+    using cd = std::complex<double>;
+    const double lp = p->lambda_p_um;
+    const bool ell = p->waist_y_um > 0;
+    const double wx = p->waist_um, wy = ell ? p->waist_y_um : p->waist_um;
+    const double w2x = wx * wx, w2y = wy * wy;
+    const cd Mx(1.0, -p->focal_um / (kPi * np * w2x / lp));
+    const cd My(1.0, -p->focal_um / (kPi * np * w2y / lp));
+    const cd cx = 1.0 / (w2x * Mx), cy = 1.0 / (w2y * My);
+    cd amp;
+    if (ell)
+        amp = std::sqrt(4.0 * p->power_W / (kEps0 * kC * kPi * np * wx * wy)) / (std::sqrt(Mx) * std::sqrt(My));
+    else
+        amp = std::sqrt(4.0 * p->power_W / (kEps0 * kC * kPi * np * w2x)) / Mx;
+    return PumpCoef{make_double2(cx.real(), cx.imag()), make_double2(cy.real(), cy.imag()),
+                    make_double2(amp.real(), amp.imag())};
+}
 }  // namespace shg
 
 using namespace shg;
@@ -122,6 +147,10 @@ int shg_validate(const shg_params* p) {
         set_error("waist must be > 0, power >= 0, focus finite");
         return SHG_EINVAL;
     }
+    if (!(p->waist_y_um >= 0) || !std::isfinite(p->waist_y_um)) {
+        set_error("waist_y_um must be 0 (circular beam) or > 0");
+        return SHG_EINVAL;
+    }
     if (!(p->lambda_p_um > 0.3) || !(p->lambda_p_um < 5.0)) {
         set_error("lambda_p=%g out of range", p->lambda_p_um);
         return SHG_EINVAL;
@@ -213,6 +242,8 @@ int shg_handle_destroy(shg_handle* h) {
     if (h->d_partials) (void)hipFree(h->d_partials);
     if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
     heat_cache_free(h);
+    for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (h->step_counter) (void)hipFree(h->step_counter);
     for (auto ev : h->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (h->stream2) {

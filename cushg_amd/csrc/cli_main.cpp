@@ -11,7 +11,8 @@
 //   --crystal=spplt|ppln  --thermal=0|1  --precision=fp32|fp64
 //   --heat=jacobi|direct  --heat-bc=open|oven  --scheme=reference|strang  --max-outer=
 //   --deff= (um/V) --period= (um) --alpha-p= --alpha-s= (1/um) --beta-s= (um/W)
-//   --rho-p= --rho-s= (rad) --out=DIR (default: current directory)
+//   --rho-p= --rho-s= (rad) --waist-y=UM (elliptical pump; default circular)
+//   --out=DIR (default: current directory)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -86,6 +87,8 @@ int main(int argc, char** argv) {
             p.rho_p = std::atof(v);
         } else if (opt(a, "--rho-s", &v)) {
             p.rho_s = std::atof(v);
+        } else if (opt(a, "--waist-y", &v)) {
+            p.waist_y_um = std::atof(v);
         } else if (opt(a, "--out", &v)) {
             out = v;
         } else {

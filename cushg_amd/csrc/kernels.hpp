@@ -49,23 +49,50 @@ struct MixedInst {
     int hist_g0, hist_n;
 };
 
+// Step index of a fused-pass launch: g, plus the value of a device counter
+// when `dev` is set (a captured graph of several steps is replayed with the
+// counter advanced by a kernel at its end, so one graph serves every chunk
+// of the march).
+struct Step {
+    const int* dev = nullptr;
+    int g = 0;
+    __host__ __device__ Step(int g_) : g(g_) {}
+    __host__ __device__ Step(const int* d, int g_) : dev(d), g(g_) {}
+    __device__ __forceinline__ int get() const { return dev ? *dev + g : g; }
+};
+
 // fused passes (kernels_mixed*.hip)
 hipError_t launch_rowpass(int nx, bool thermal, Arith a, int rows_pw, dim3 grid, hipStream_t st,
-                          const MixedInst* insts, int g, const double2* tw);
+                          const MixedInst* insts, Step g, const double2* tw);
 hipError_t launch_colpass(int ny, Arith a, int cols_pw, dim3 grid, hipStream_t st, const MixedInst* insts,
-                          int g, const double2* tw);
+                          Step g, const double2* tw);
+// row-pass plan value selecting the wide form (NX threads per row, one point
+// each in the nonlinear step; rows per workgroup fixed at max(1, 256/NX))
+constexpr int kRowsWide = SHG_PLAN_ROWS_WIDE;
+int rowpass_rows_per_wg(int nx, int rows);  // rows one workgroup covers under a plan value
 int rowpass_rows_default(int nx, Arith a);
+int colpass_cols_auto(int ny, Arith a, long lines);  // by launch size
 int colpass_cols_default(int ny, Arith a);
 bool colpass_cols_supported(int ny, int cols);
 bool rowpass_rows_supported(int nx, int rows);
 
+// Initial pump A(x, y) = amp exp(-x^2 cx - y^2 cy) (setPump,
+// src/headers/Efields.h:84-119): with M = 1 - i f / zR per axis,
+// cx = 1 / (wx^2 Mx), cy = 1 / (wy^2 My), amp = Ap0 / M for the circular
+// beam and Ap0' / sqrt(Mx My) for the elliptical one.
+struct PumpCoef {
+    double2 cx, cy, amp;
+};
+PumpCoef pump_coef(const shg_params* p, double n_p);
+
 // utility kernels (kernels_util.hip)
 template <typename C>
-hipError_t launch_pump_init(hipStream_t st, C* P, int nx, int ny, double dx, double dy, double w2,
-                            double Ap0, double Mre, double Mim);
+hipError_t launch_pump_init(hipStream_t st, C* P, int nx, int ny, double dx, double dy, const PumpCoef& c);
 hipError_t launch_sumsq_f32(hipStream_t st, const float2* A, long n, double* partials, int nblocks);
 hipError_t launch_sumsq_f64(hipStream_t st, const double2* A, long n, double* partials, int nblocks);
 hipError_t launch_sum_partials(hipStream_t st, const double* partials, int n, double* out);
 hipError_t launch_f64_to_f32(hipStream_t st, const double* in, float* out, long n);
+hipError_t launch_step_set(hipStream_t st, int* counter, int value);
+hipError_t launch_step_advance(hipStream_t st, int* counter, int by);
 
 }  // namespace shg

@@ -8,40 +8,33 @@ namespace shg {
 //   A = (Ap0/M) exp(-(rx^2 + ry^2) / (w0^2 M)),  M = 1 - i f/zR,
 //   rx = (x - nx/2) dx,  ry = (y - ny/2) dy
 template <typename C>
-__global__ void k_pump_init(C* __restrict__ P, int nx, int ny, double dx, double dy, double w2,
-                            double Ap0, double Mre, double Mim) {
+__global__ void k_pump_init(C* __restrict__ P, int nx, int ny, double dx, double dy, PumpCoef c) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= nx || y >= ny) return;
     const double rx = (x - 0.5 * nx) * dx, ry = (y - 0.5 * ny) * dy;
-    const double m2 = Mre * Mre + Mim * Mim;
-    // -(r^2)/(w2 M) = -(r^2) conj(M) / (w2 |M|^2)
-    const double a = -(rx * rx + ry * ry) / (w2 * m2);
-    const double er = a * Mre, ei = -a * Mim;
+    // amp * exp(-rx^2 cx - ry^2 cy), cx = 1/(wx^2 Mx), cy = 1/(wy^2 My)
+    const double er = -(rx * rx) * c.cx.x - (ry * ry) * c.cy.x;
+    const double ei = -(rx * rx) * c.cx.y - (ry * ry) * c.cy.y;
     const double mag = exp(er);
     double sn, cs;
     sincos(ei, &sn, &cs);
-    // Ap0 / M = Ap0 conj(M) / |M|^2
-    const double pr = Ap0 * Mre / m2, pi = -Ap0 * Mim / m2;
     const double ar = mag * cs, ai = mag * sn;
     C v;
-    v.x = (decltype(v.x))(pr * ar - pi * ai);
-    v.y = (decltype(v.y))(pr * ai + pi * ar);
+    v.x = (decltype(v.x))(c.amp.x * ar - c.amp.y * ai);
+    v.y = (decltype(v.y))(c.amp.x * ai + c.amp.y * ar);
     P[(size_t)y * nx + x] = v;
 }
 
 template <typename C>
-hipError_t launch_pump_init(hipStream_t st, C* P, int nx, int ny, double dx, double dy, double w2,
-                            double Ap0, double Mre, double Mim) {
+hipError_t launch_pump_init(hipStream_t st, C* P, int nx, int ny, double dx, double dy, const PumpCoef& c) {
     dim3 block(64, 4);
     dim3 grid((nx + 63) / 64, (ny + 3) / 4);
-    hipLaunchKernelGGL((k_pump_init<C>), grid, block, 0, st, P, nx, ny, dx, dy, w2, Ap0, Mre, Mim);
+    hipLaunchKernelGGL((k_pump_init<C>), grid, block, 0, st, P, nx, ny, dx, dy, c);
     return hipGetLastError();
 }
-template hipError_t launch_pump_init<float2>(hipStream_t, float2*, int, int, double, double, double,
-                                             double, double, double);
-template hipError_t launch_pump_init<double2>(hipStream_t, double2*, int, int, double, double, double,
-                                              double, double, double);
+template hipError_t launch_pump_init<float2>(hipStream_t, float2*, int, int, double, double, const PumpCoef&);
+template hipError_t launch_pump_init<double2>(hipStream_t, double2*, int, int, double, double, const PumpCoef&);
 
 // FP64 partial sums of |A|^2 (block-level tree reduction, deterministic)
 template <typename C>
@@ -98,6 +91,25 @@ hipError_t launch_f64_to_f32(hipStream_t st, const double* in, float* out, long 
     long nb = (n + 255) / 256;
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)nb), dim3(256), 0, st, in, out, n);
+    return hipGetLastError();
+}
+
+// step counter of graph-replayed marches: set, and advance by a chunk
+// (one thread; a fill kernel rather than a memset so that it may be captured)
+__global__ void k_step_set(int* c, int v) {
+    if (threadIdx.x == 0) *c = v;
+}
+__global__ void k_step_advance(int* c, int by) {
+    if (threadIdx.x == 0) *c += by;
+}
+
+hipError_t launch_step_set(hipStream_t st, int* counter, int value) {
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, st, counter, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(hipStream_t st, int* counter, int by) {
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, counter, by);
     return hipGetLastError();
 }
 

@@ -48,9 +48,9 @@ int plan_rows(shg_handle* h, int nx, Arith a) {
     auto it = h->rows_plan.find({nx, (int)a});
     return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx, a);
 }
-int plan_cols(shg_handle* h, int ny, Arith a) {
+int plan_cols(shg_handle* h, int ny, Arith a, long lines) {
     auto it = h->cols_plan.find({ny, (int)a});
-    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_default(ny, a);
+    return it != h->cols_plan.end() && it->second > 0 ? it->second : colpass_cols_auto(ny, a, lines);
 }
 
 int field_sum_sq(shg_handle* h, Arith a, const void* A, long n, double* slot) {
@@ -182,18 +182,13 @@ int setup_fields(shg_handle* h, MixedHost& m) {
     // the host vectors die at return: make the copies complete first
     SHG_HIP(hipStreamSynchronize(h->stream));
     const shg_params* p = m.p;
-    const double np = refr_index<double>(p->crystal, p->lambda_p_um, m.Tc);
-    const double w2 = p->waist_um * p->waist_um;
-    const double zR = kPi * np * w2 / p->lambda_p_um;
-    const double Ap0 = std::sqrt(4.0 * p->power_W / (kEps0 * kC * kPi * np * w2));
+    const PumpCoef pc = pump_coef(p, refr_index<double>(p->crystal, p->lambda_p_um, m.Tc));
     {
         ProfScope ps(h, "pump_init");
         if (m.arith == Arith::F64)
-            SHG_HIP(launch_pump_init<double2>(h->stream, static_cast<double2*>(m.P), m.nx, m.ny, m.dx, m.dy, w2, Ap0,
-                                              1.0, -p->focal_um / zR));
+            SHG_HIP(launch_pump_init<double2>(h->stream, static_cast<double2*>(m.P), m.nx, m.ny, m.dx, m.dy, pc));
         else
-            SHG_HIP(launch_pump_init<float2>(h->stream, static_cast<float2*>(m.P), m.nx, m.ny, m.dx, m.dy, w2, Ap0,
-                                             1.0, -p->focal_um / zR));
+            SHG_HIP(launch_pump_init<float2>(h->stream, static_cast<float2*>(m.P), m.nx, m.ny, m.dx, m.dy, pc));
     }
     SHG_HIP(hipMemsetAsync(m.S, 0, arith_field_bytes(m.arith) * m.N2, h->stream));
     return field_sum_sq(h, m.arith, m.P, (long)m.N2, m.slots + 0);
@@ -210,19 +205,97 @@ static int deliver_slice(shg_handle* h, MixedHost& m, int g) {
     return t.deliver(g, m.pass);
 }
 
+// Instances sharing one launch of a pass: same transform length (and
+// THERMAL flag for the row pass) and form, descriptors at d, sorted by nz
+// descending.
+struct Group {
+    int key_n;
+    bool thermal;
+    Arith arith;
+    std::vector<MixedHost*> m;
+    MixedInst* d = nullptr;
+    const double2* tw = nullptr;
+};
+using RowGroups = std::map<std::tuple<int, bool, int>, Group>;
+using ColGroups = std::map<std::pair<int, int>, Group>;
+
+// Graph replay of the march: the launches of `chunk` consecutive steps
+// (every group's row pass, then every group's column pass, per step) and
+// a counter advance are captured once into a graph whose kernels read the
+// step index from the handle's device counter; the graph is replayed
+// ceil((G + 1) / chunk) times.  Every launch covers all instances of its
+// group (an instance past its last step returns at once), so the captured
+// shapes do not change along the march and the graph is cached on the
+// handle, keyed by those shapes and pointers, for later marches.
+static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G) {
+    if (!h->step_counter) SHG_HIP(hipMalloc(&h->step_counter, 256));
+    const int C = h->graph_chunk;
+    struct Launch {
+        bool row;
+        const Group* gr;
+        int plan;
+        dim3 grid;
+    };
+    std::vector<Launch> ls;
+    std::vector<uint64_t> key{(uint64_t)C, (uint64_t)(uintptr_t)h->step_counter};
+    auto add = [&](bool row, const Group& gr) {
+        int maxn = 0;
+        for (auto* m : gr.m) maxn = std::max(maxn, row ? m->ny : m->nx);
+        long lines = 0;
+        for (auto* m : gr.m) lines += m->nx;
+        const int plan = row ? plan_rows(h, gr.key_n, gr.arith) : plan_cols(h, gr.key_n, gr.arith, lines);
+        const int per = row ? rowpass_rows_per_wg(gr.key_n, plan) : plan;
+        const dim3 grid((maxn + per - 1) / per, row ? 1 : 2, (unsigned)gr.m.size());
+        ls.push_back({row, &gr, plan, grid});
+        for (uint64_t v : {(uint64_t)row, (uint64_t)gr.key_n, (uint64_t)gr.thermal, (uint64_t)gr.arith,
+                           (uint64_t)plan, (uint64_t)grid.x, (uint64_t)grid.z, (uint64_t)(uintptr_t)gr.d,
+                           (uint64_t)(uintptr_t)gr.tw})
+            key.push_back(v);
+    };
+    for (auto& kv : rowg) add(true, kv.second);
+    for (auto& kv : colg) add(false, kv.second);
+    hipGraphExec_t ex = nullptr;
+    auto it = h->graphs.find(key);
+    if (it != h->graphs.end()) {
+        ex = it->second;
+    } else {
+        if (h->graphs.size() >= 16) {
+            for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+            h->graphs.clear();
+        }
+        SHG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int c = 0; c < C && e == hipSuccess; c++)
+            for (const Launch& l : ls) {
+                const Step st(h->step_counter, c);
+                e = l.row ? launch_rowpass(l.gr->key_n, l.gr->thermal, l.gr->arith, l.plan, l.grid, h->stream,
+                                           l.gr->d, st, l.gr->tw)
+                          : launch_colpass(l.gr->key_n, l.gr->arith, l.plan, l.grid, h->stream, l.gr->d, st,
+                                           l.gr->tw);
+                if (e != hipSuccess) break;
+            }
+        if (e == hipSuccess) e = launch_step_advance(h->stream, h->step_counter, C);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            set_error("march graph capture: %s", hipGetErrorString(e));
+            return SHG_EDEVICE;
+        }
+        h->graphs[key] = ex;
+    }
+    SHG_HIP(launch_step_set(h->stream, h->step_counter, 0));
+    for (int r = 0; r < (G + C) / C; r++) SHG_HIP(hipGraphLaunch(ex, h->stream));
+    return SHG_OK;
+}
+
 // The batched z-march: launches A(0) B(0) A(1) ... B(G-1) A(G), G = max nz.
 static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, size_t pool_cap) {
     size_t pool_used = 0;
-    struct Group {
-        int key_n;
-        bool thermal;
-        Arith arith;
-        std::vector<MixedHost*> m;
-        MixedInst* d = nullptr;
-        const double2* tw = nullptr;
-    };
-    std::map<std::tuple<int, bool, int>, Group> rowg;
-    std::map<std::pair<int, int>, Group> colg;
+    RowGroups rowg;
+    ColGroups colg;
     for (auto* m : act) {
         auto& rg = rowg[{m->nx, m->thermal, (int)m->arith}];
         rg.key_n = m->nx;
@@ -262,6 +335,13 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         if (int rc = upload(kv.second)) return rc;
     int G = 0;
     for (auto* m : act) G = std::max(G, m->nz);
+    const bool traced = std::any_of(act.begin(), act.end(), [](const MixedHost* m) { return m->trace != nullptr; });
+    if (h->graph_chunk > 0 && !h->profiling && !traced) {
+        if (int rc = march_graph(h, rowg, colg, G)) return rc;
+        SHG_HIP(hipStreamSynchronize(h->stream));
+        for (auto* m : act) m->steps += m->nz;
+        return SHG_OK;
+    }
     for (int g = 0; g <= G; g++) {
         for (auto& kv : rowg) {
             Group& gr = kv.second;
@@ -273,7 +353,8 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             }
             if (!cnt) continue;
             const int rows = plan_rows(h, gr.key_n, gr.arith);
-            dim3 grid((maxny + rows - 1) / rows, 1, cnt);
+            const int per = rowpass_rows_per_wg(gr.key_n, rows);
+            dim3 grid((maxny + per - 1) / per, 1, cnt);
             ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith));
             SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, h->stream, gr.d, g, gr.tw));
         }
@@ -284,13 +365,15 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         for (auto& kv : colg) {
             Group& gr = kv.second;
             int cnt = 0, maxnx = 0;
+            long lines = 0;
             for (auto* m : gr.m) {
                 if (m->nz <= g) break;
                 cnt++;
                 maxnx = std::max(maxnx, m->nx);
+                lines += m->nx;
             }
             if (!cnt) continue;
-            const int cols = plan_cols(h, gr.key_n, gr.arith);
+            const int cols = plan_cols(h, gr.key_n, gr.arith, lines);
             dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
             ProfScope ps(h, colpass_name(gr.key_n, gr.arith));
             SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, h->stream, gr.d, g, gr.tw));

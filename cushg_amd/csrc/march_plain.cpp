@@ -175,10 +175,7 @@ int run_plain(shg_handle* h, const shg_params* p, shg_result* r, SliceTrace* tra
         SHG_HIP(hipMemcpy(m.propS, ps.data(), sizeof(double2) * m.n2, hipMemcpyHostToDevice));
         const double np = refr_index<double>(p->crystal, lp, Tc);
         const double ns = refr_index<double>(p->crystal, ls, Tc);
-        const double w2 = p->waist_um * p->waist_um;
-        const double zR = kPi * np * w2 / lp;
-        const double Ap0 = std::sqrt(4.0 * p->power_W / (kEps0 * kC * kPi * np * w2));
-        SHG_HIP(launch_pump_init<double2>(h->stream, m.P, m.nx, m.ny, m.dx, m.dy, w2, Ap0, 1.0, -p->focal_um / zR));
+        SHG_HIP(launch_pump_init<double2>(h->stream, m.P, m.nx, m.ny, m.dx, m.dy, pump_coef(p, np)));
         SHG_HIP(hipMemsetAsync(m.S, 0, sizeof(double2) * m.n2, h->stream));
         if (int rc = sum_sq64(h, m.P, m.n2, m.slots + 0)) return rc;
 

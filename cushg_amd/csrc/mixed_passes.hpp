@@ -139,98 +139,120 @@ __device__ __forceinline__ int heat_count(int h, int nz, int nzh) {
     return (int)(a - b);
 }
 
-// Nonlinear work of the row pass at global step g on the 8 points
-// x = tp + q*TPR of row y: R call 2 of slice g-1 (if g > 0; with THERMAL
-// the heat source of slice g-1 is deposited after it; the state is then
-// saved and traced as asked), then R call 1 of slice g (if g < nz).  Both
-// calls run point by point, so a point's heat-voxel values are loaded once
-// and the second call's phase factor is the first's times a small-angle
-// factor exp(i (phi_g - phi_{g-1})).
-template <int NX, bool THERMAL, typename S, typename C>
-__device__ __forceinline__ void nl_pair(C (&p)[8], C (&s)[8], const MixedInst& I, int g, int y, int tp,
-                                        size_t base, bool trace) {
+// Nonlinear work of the row pass at global step g on the points of row y:
+// R call 2 of slice g-1 (if g > 0; with THERMAL the heat source of slice
+// g-1 is deposited after it; the state is then saved and traced as asked),
+// then R call 1 of slice g (if g < nz).  Both calls run point by point, so
+// a point's heat-voxel values are loaded once and the second call's phase
+// factor is the first's times a small-angle factor exp(i (phi_g - phi_{g-1})).
+// The constructor holds what is uniform over the row; point() does one
+// point (column x, element index i of the field).
+template <bool THERMAL, typename S, typename C> struct NlRow {
     using R = re_t<C>;
-    constexpr int TPR = NX / 8;
-    const int nz = I.nz;
-    const bool c1 = g > 0, c0 = g < nz;
-    const bool save = c1 && g - 1 == nz - 2;
-    const RkConst<R> k{(R)I.c1, (R)I.c2, (R)I.c3, (R)I.w, (R)I.ap_h, (R)I.as_h};
-    // saved output and trace share a key: a traced slice nz-1 is that state, bit for bit
-    const uint32_t kop = store_key(2, 0, g), kos = store_key(2, 1, g);
-    auto emit = [&](int q) {
-        const size_t i = base + q * TPR;
+    const MixedInst& I;
+    int g, nz;
+    bool c1, c0, save, trace;
+    RkConst<R> k;
+    uint32_t kop, kos;
+    // non-THERMAL: the two calls' phase factors and couplings
+    C e1[3], e0[3];
+    R kp, ks, bfac;
+    // THERMAL: heat-voxel rows of the two calls and the phase scales
+    size_t row1 = 0, row0 = 0;
+    int h1 = 0, h0 = 0;
+    double zsc1 = 0, zsc0 = 0, o1 = 0, o0 = 0, pol1 = 0, pol0 = 0;
+    bool first = false;
+    R qw = 0, kp0 = 0, ks0 = 0, bf0 = 0, hce = 0;
+
+    __device__ __forceinline__ NlRow(const MixedInst& I_, int g_, int y, bool trace_) : I(I_), g(g_), nz(I_.nz) {
+        c1 = g > 0;
+        c0 = g < nz;
+        save = c1 && g - 1 == nz - 2;
+        trace = trace_;
+        k = RkConst<R>{(R)I.c1, (R)I.c2, (R)I.c3, (R)I.w, (R)I.ap_h, (R)I.as_h};
+        // saved output and trace share a key: a traced slice nz-1 is that state, bit for bit
+        kop = store_key(2, 0, g);
+        kos = store_key(2, 1, g);
+        if constexpr (!THERMAL) {
+            // phase table: exp(i phi) of (slice, call, offset), FP64
+            const double2* t1 = I.phase + (size_t)(max(g - 1, 0) * 2 + 1) * 3;
+            const double2* t0 = I.phase + (size_t)(min(g, nz - 1) * 2) * 3;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                e1[j] = c1 ? mk<C>((R)t1[j].x, (R)t1[j].y) : C{};
+                e0[j] = c0 ? mk<C>((R)t0[j].x, (R)t0[j].y) : C{};
+            }
+            kp = (R)I.kp;
+            ks = (R)I.ks;
+            bfac = (R)I.bfac;
+        } else {
+            const size_t plane = (size_t)I.nx * I.ny;
+            // heat slices of the two calls (clamped to a valid slice when a call does not run)
+            h1 = heat_slice(c1 ? g - 1 : g, nz, I.nzh);
+            h0 = heat_slice(c0 ? g : g - 1, nz, I.nzh);
+            row1 = (size_t)h1 * plane + (size_t)y * I.nx;
+            row0 = (size_t)h0 * plane + (size_t)y * I.nx;
+            // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50,
+            // with DKint the running dk integral (src/headers/PhaseMatching.h:27-44)
+            zsc1 = I.dz / (double)g;
+            zsc0 = I.dz / ((double)g + 1.0);
+            o1 = (double)(g - 1) + I.off[1][0];
+            o0 = (double)g + I.off[0][0];
+            pol1 = c1 && I.poling ? I.poling[g - 1] : 0.0;
+            pol0 = c0 && I.poling ? I.poling[g] : 0.0;
+            first = g - 1 == 0 || (c1 && heat_slice(g - 2, nz, I.nzh) != h1);
+            qw = c1 ? R(1) / (R)heat_count(h1, nz, I.nzh) : R(0);
+            kp0 = (R)I.kp0;
+            ks0 = (R)I.ks0;
+            bf0 = (R)I.bf0;
+            hce = (R)I.hce;
+        }
+    }
+
+    __device__ __forceinline__ void emit(const C& p, const C& s, size_t i) const {
         if (save) {
-            static_cast<S*>(I.outP)[i] = to_store<S>(p[q], i, kop);
-            static_cast<S*>(I.outS)[i] = to_store<S>(s[q], i, kos);
+            static_cast<S*>(I.outP)[i] = to_store<S>(p, i, kop);
+            static_cast<S*>(I.outS)[i] = to_store<S>(s, i, kos);
         }
         if (trace) {
-            static_cast<S*>(I.trP)[i] = to_store<S>(p[q], i, kop);
-            static_cast<S*>(I.trS)[i] = to_store<S>(s[q], i, kos);
+            static_cast<S*>(I.trP)[i] = to_store<S>(p, i, kop);
+            static_cast<S*>(I.trS)[i] = to_store<S>(s, i, kos);
         }
-    };
-    if constexpr (!THERMAL) {
-        // phase table: exp(i phi) of (slice, call, offset), FP64
-        C e1[3], e0[3];
-        const double2* t1 = I.phase + (size_t)(max(g - 1, 0) * 2 + 1) * 3;
-        const double2* t0 = I.phase + (size_t)(min(g, nz - 1) * 2) * 3;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            e1[j] = c1 ? mk<C>((R)t1[j].x, (R)t1[j].y) : C{};
-            e0[j] = c0 ? mk<C>((R)t0[j].x, (R)t0[j].y) : C{};
-        }
-        const R kp = (R)I.kp, ks = (R)I.ks, bfac = (R)I.bfac;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
+    }
+
+    __device__ __forceinline__ void point(C& p, C& s, int x, size_t i) const {
+        if constexpr (!THERMAL) {
             if (c1) {
-                rk4_point(p[q], s[q], e1, kp, ks, bfac, k);
-                emit(q);
+                rk4_point(p, s, e1, kp, ks, bfac, k);
+                emit(p, s, i);
             }
-            if (c0) rk4_point(p[q], s[q], e0, kp, ks, bfac, k);
-        }
-    } else {
-        const size_t plane = (size_t)I.nx * I.ny;
-        // heat slices of the two calls (clamped to a valid slice when a call does not run)
-        const int h1 = heat_slice(c1 ? g - 1 : g, nz, I.nzh);
-        const int h0 = heat_slice(c0 ? g : g - 1, nz, I.nzh);
-        const size_t row1 = (size_t)h1 * plane + (size_t)y * I.nx;
-        const size_t row0 = (size_t)h0 * plane + (size_t)y * I.nx;
-        // phase DKint (s + off) dz / (s + 1) of src/headers/Solver.h:48,50,
-        // with DKint the running dk integral (src/headers/PhaseMatching.h:27-44)
-        const double zsc1 = I.dz / (double)g, zsc0 = I.dz / ((double)g + 1.0);
-        const double o1 = (double)(g - 1) + I.off[1][0], o0 = (double)g + I.off[0][0];
-        const double pol1 = c1 && I.poling ? I.poling[g - 1] : 0.0;
-        const double pol0 = c0 && I.poling ? I.poling[g] : 0.0;
-        const bool first = g - 1 == 0 || (c1 && heat_slice(g - 2, nz, I.nzh) != h1);
-        const R qw = c1 ? R(1) / (R)heat_count(h1, nz, I.nzh) : R(0);
-        const R kp0 = (R)I.kp0, ks0 = (R)I.ks0, bf0 = (R)I.bf0, hce = (R)I.hce;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int x = tp + q * TPR;
+            if (c0) rk4_point(p, s, e0, kp, ks, bfac, k);
+        } else {
             const size_t v = (c1 ? row1 : row0) + x;
             float2 n = I.nn[v];  // n_p(T), n_s(T) of the voxel
             double dkb = I.dkbase[v];
             double dkv = I.dk[v];
             R np = n.x, ns = n.y;
-            R kp = kp0 * recip<R>(n.x), ks = ks0 * recip<R>(n.y), bfac = bf0 * ns;
-            C e0 = C{};
+            R kpv = kp0 * recip<R>(n.x), ksv = ks0 * recip<R>(n.y), bfv = bf0 * ns;
+            C e0v = C{};
             double ph = 0.0;
             if (c1) {
                 const double dki = dkb + (double)(g - 1) * dkv;
                 ph = dki * o1 * zsc1 + pol1;
                 // the three offsets are 0.25 dz apart in both schemes:
                 // e(off + 1/4) = e(off) exp(i DKint dz / (4 (s + 1)))
-                e0 = expi<C>(ph);
+                e0v = expi<C>(ph);
                 const C ed = expi_small<C>(0.25 * dki * zsc1);
-                const C e1 = cmul(e0, ed);
-                const C e[3] = {e0, e1, cmul(e1, ed)};
-                rk4_point(p[q], s[q], e, kp, ks, bfac, k);
+                const C e1v = cmul(e0v, ed);
+                const C e[3] = {e0v, e1v, cmul(e1v, ed)};
+                rk4_point(p, s, e, kpv, ksv, bfv, k);
                 // Q = a_p I_p + a_s I_s + b_s I_s^2 -- src/headers/Tfield.h:260-276
-                const R Ip = hce * np * abs2(p[q]);
-                const R Is = hce * ns * abs2(s[q]);
+                const R Ip = hce * np * abs2(p);
+                const R Is = hce * ns * abs2(s);
                 const R qv = ((R)I.alpha_p * Ip + (R)I.alpha_s * Is + (R)I.beta_s * Is * Is) * qw;
                 float* Qp = I.Q + row1 + x;
                 *Qp = first ? (float)qv : (float)((R)*Qp + qv);
-                emit(q);
+                emit(p, s, i);
             }
             if (c0) {
                 if (c1 && h0 != h1) {  // the call falls in the next heat slab
@@ -239,20 +261,30 @@ __device__ __forceinline__ void nl_pair(C (&p)[8], C (&s)[8], const MixedInst& I
                     dkv = I.dk[row0 + x];
                     np = n.x;
                     ns = n.y;
-                    kp = kp0 * recip<R>(n.x);
-                    ks = ks0 * recip<R>(n.y);
-                    bfac = bf0 * ns;
+                    kpv = kp0 * recip<R>(n.x);
+                    ksv = ks0 * recip<R>(n.y);
+                    bfv = bf0 * ns;
                 }
                 const double dki = dkb + (double)g * dkv;
                 const double ph0 = dki * o0 * zsc0 + pol0;
-                e0 = c1 ? cmul(e0, expi_small<C>(ph0 - ph)) : expi<C>(ph0);
+                e0v = c1 ? cmul(e0v, expi_small<C>(ph0 - ph)) : expi<C>(ph0);
                 const C ed = expi_small<C>(0.25 * dki * zsc0);
-                const C e1 = cmul(e0, ed);
-                const C e[3] = {e0, e1, cmul(e1, ed)};
-                rk4_point(p[q], s[q], e, kp, ks, bfac, k);
+                const C e1v = cmul(e0v, ed);
+                const C e[3] = {e0v, e1v, cmul(e1v, ed)};
+                rk4_point(p, s, e, kpv, ksv, bfv, k);
             }
         }
     }
+};
+
+// the row's 8 points x = tp + q*TPR of one thread
+template <int NX, bool THERMAL, typename S, typename C>
+__device__ __forceinline__ void nl_pair(C (&p)[8], C (&s)[8], const MixedInst& I, int g, int y, int tp,
+                                        size_t base, bool trace) {
+    constexpr int TPR = NX / 8;
+    const NlRow<THERMAL, S, C> nl(I, g, y, trace);
+#pragma unroll
+    for (int q = 0; q < 8; q++) nl.point(p[q], s[q], tp + q * TPR, base + q * TPR);
 }
 
 // Row pass at global step g of every instance in the launch group.
@@ -280,7 +312,8 @@ template <typename C> struct RowWaves { static constexpr int W = sizeof(C) == 16
 
 template <int NX, bool THERMAL, int ROWS, typename S, typename C>
 __global__ __launch_bounds__(NX / 8 * ROWS) __attribute__((amdgpu_waves_per_eu(RowWaves<C>::W)))
-void k_rowpass(const MixedInst* __restrict__ insts, int g, const double2* __restrict__ tw) {
+void k_rowpass(const MixedInst* __restrict__ insts, Step gs, const double2* __restrict__ tw) {
+    const int g = gs.get();
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
     __shared__ C lds[RowLds<C>::BUFS * ROWS * LD];
@@ -324,12 +357,101 @@ void k_rowpass(const MixedInst* __restrict__ insts, int g, const double2* __rest
     }
 }
 
+// Wide row pass: the same step as k_rowpass with NX threads per row, for
+// launches too small to fill the chip with NX/8 threads per row (a single
+// grid of 256 x 256 or less, or a small batch).  The transforms run on
+// groups of NX/8 threads holding 8 points each (group 2r + f: row r, field
+// f; the other groups only join the barriers), the nonlinear work runs one
+// point per thread from LDS, which cuts the serial work of a thread by 8.
+// Bit-identical to k_rowpass (same transforms, same per-point arithmetic,
+// same stores).  WROWS = max(1, 256 / NX) rows per workgroup.
+template <int NX> constexpr int wide_rows() { return NX >= 256 ? 1 : 256 / NX; }
+
+template <int NX, bool THERMAL, typename S, typename C>
+__global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const MixedInst* __restrict__ insts, Step gs,
+                                                                        const double2* __restrict__ tw) {
+    constexpr int ROWS = wide_rows<NX>();
+    constexpr int TPR = NX / 8;
+    constexpr int LD = FftShape<NX>::LDS;
+    __shared__ C lds[2 * ROWS * LD];
+    const int g = gs.get();
+    const MixedInst& I = insts[blockIdx.z];
+    const int nz = I.nz;
+    if (g > nz) return;
+    if (blockIdx.x * ROWS >= I.ny) return;
+    const int t = threadIdx.x, gi = t / TPR, tp = t % TPR;
+    const size_t row0 = (size_t)blockIdx.x * ROWS * NX;
+    S* const P = static_cast<S*>(I.P);
+    S* const Sf = static_cast<S*>(I.S);
+    // per-slice trace: the state before the SSFM step of slice g
+    const bool trace = I.trP && g <= nz - 1 && g % I.trace_every == 0;
+    // 1. rows in, spatial (row IFFT unless g == 0), into LDS
+    {
+        const bool act = gi < 2 * ROWS;
+        const int f = gi & 1;
+        const size_t base = row0 + (size_t)(gi >> 1) * NX + tp;
+        S* const F = f ? Sf : P;
+        C v[8];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = from_store<C>(F[base + q * TPR]);
+            if (trace && g == 0) {
+                S* const T = static_cast<S*>(f ? I.trS : I.trP);
+                const uint32_t key = store_key(3, f, g);
+#pragma unroll
+                for (int q = 0; q < 8; q++) T[base + q * TPR] = to_store<S>(v[q], base + q * TPR, key);
+            }
+        }
+        C* buf = lds + (act ? gi : 0) * LD;
+        if (g > 0) fft_dist<C, NX, true>(v, buf, tw, tp, act);
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) buf[fpad(tp + q * TPR)] = v[q];
+        }
+    }
+    __syncthreads();
+    // 2. one point per thread
+    {
+        const int r = t / NX, x = t % NX;
+        C* rb = lds + 2 * r * LD;
+        const NlRow<THERMAL, S, C> nl(I, g, blockIdx.x * ROWS + r, trace);
+        C p = rb[fpad(x)], sv = rb[LD + fpad(x)];
+        nl.point(p, sv, x, row0 + (size_t)r * NX + x);
+        rb[fpad(x)] = p;
+        rb[LD + fpad(x)] = sv;
+    }
+    __syncthreads();
+    // 3. row FFT (unless g == nz) and out
+    {
+        const bool act = gi < 2 * ROWS;
+        const int f = gi & 1;
+        C* buf = lds + (act ? gi : 0) * LD;
+        C v[8];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = buf[fpad(tp + q * TPR)];
+        }
+        if (g < nz) {
+            __syncthreads();  // every group has read its buffer before the transform overwrites it
+            fft_dist<C, NX, false>(v, buf, tw, tp, act);
+        }
+        if (act) {
+            S* const F = f ? Sf : P;
+            const uint32_t key = store_key(0, f, g);
+            const size_t base = row0 + (size_t)(gi >> 1) * NX + tp;
+#pragma unroll
+            for (int q = 0; q < 8; q++) F[base + q * TPR] = to_store<S>(v[q], base + q * TPR, key);
+        }
+    }
+}
+
 // Column pass of step g: column FFT -> * Prop -> column IFFT on COLS
 // adjacent columns of one field (blockIdx.y: 0 pump, 1 SH).
 // grid (ceil(max_nx/COLS), 2, n_instances), block (COLS, NY/8)
 template <int NY, int COLS, typename S, typename C>
-__global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __restrict__ insts, int g,
+__global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __restrict__ insts, Step gs,
                                                            const double2* __restrict__ tw) {
+    const int g = gs.get();
     using R = re_t<C>;
     constexpr int TPC = NY / 8;
     constexpr int LD = FftShape<NY>::LDS;
@@ -395,10 +517,13 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __re
 // pair; each pair is instantiated in its own translation unit
 // ------------------------------------------------------------------
 template <int NX, bool TH, typename S, typename C>
-static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
+static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedInst* insts, Step g,
                                const double2* tw) {
     dim3 block(NX / 8, rows);
     switch (rows) {
+        case kRowsWide:
+            hipLaunchKernelGGL((k_rowpass_wide<NX, TH, S, C>), grid, dim3(NX * wide_rows<NX>()), 0, st, insts, g, tw);
+            break;
         case 1: hipLaunchKernelGGL((k_rowpass<NX, TH, 1, S, C>), grid, block, 0, st, insts, g, tw); break;
         case 2: hipLaunchKernelGGL((k_rowpass<NX, TH, 2, S, C>), grid, block, 0, st, insts, g, tw); break;
         case 4: hipLaunchKernelGGL((k_rowpass<NX, TH, 4, S, C>), grid, block, 0, st, insts, g, tw); break;
@@ -415,7 +540,7 @@ static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedI
 
 template <typename S, typename C>
 static hipError_t rowpass_dispatch(int nx, bool thermal, int rows, dim3 grid, hipStream_t st,
-                                   const MixedInst* insts, int g, const double2* tw) {
+                                   const MixedInst* insts, Step g, const double2* tw) {
 #define SHG_ROWCASE(N)                                                                         \
     case N:                                                                                    \
         return thermal ? rowpass_rows<N, true, S, C>(rows, grid, st, insts, g, tw)             \
@@ -433,10 +558,11 @@ static hipError_t rowpass_dispatch(int nx, bool thermal, int rows, dim3 grid, hi
 }
 
 template <int NY, typename S, typename C>
-static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
+static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedInst* insts, Step g,
                                const double2* tw) {
     dim3 block(cols, NY / 8);
     switch (cols) {
+        case 2: hipLaunchKernelGGL((k_colpass<NY, 2, S, C>), grid, block, 0, st, insts, g, tw); break;
         case 4: hipLaunchKernelGGL((k_colpass<NY, 4, S, C>), grid, block, 0, st, insts, g, tw); break;
         case 8: hipLaunchKernelGGL((k_colpass<NY, 8, S, C>), grid, block, 0, st, insts, g, tw); break;
         case 16:
@@ -451,7 +577,7 @@ static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedI
 }
 
 template <typename S, typename C>
-static hipError_t colpass_dispatch(int ny, int cols, dim3 grid, hipStream_t st, const MixedInst* insts, int g,
+static hipError_t colpass_dispatch(int ny, int cols, dim3 grid, hipStream_t st, const MixedInst* insts, Step g,
                                    const double2* tw) {
     switch (ny) {
         case 64: return colpass_cols<64, S, C>(cols, grid, st, insts, g, tw);

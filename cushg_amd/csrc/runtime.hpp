@@ -89,6 +89,12 @@ struct shg_handle {
     // columns per workgroup (column pass, by (ny, form)); absent = built-in
     // default
     std::map<std::pair<int, int>, int> rows_plan, cols_plan;
+    // graph replay of the fused march: steps per captured graph (0: direct
+    // launches), the device step counter and the cached executable graphs
+    // keyed by the launch structure they were captured from
+    int graph_chunk = 0;
+    int* step_counter = nullptr;
+    std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
     // profiling
     bool profiling = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;

@@ -30,7 +30,7 @@ EXPORTS = (
     "shg_refractive_index", "shg_delta_k", "shg_validate", "shg_last_error",
     "shg_handle_create", "shg_handle_destroy", "shg_run", "shg_run_traced", "shg_run_batch", "shg_adjoint",
     "shg_cli_main", "shg_last_kernel_times", "shg_handle_set_profiling", "shg_set_plan",
-    "shg_autotune",
+    "shg_get_plan", "shg_autotune", "shg_set_graph_steps",
 )
 
 _lib = None
@@ -79,7 +79,9 @@ def lib():
                                         C.POINTER(C.c_int32), C.c_int32]
     L.shg_handle_set_profiling.argtypes = [H, C.c_int32]
     L.shg_set_plan.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.shg_get_plan.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.shg_autotune.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.shg_set_graph_steps.argtypes = [H, C.c_int32]
     _lib = L
     return L
 
@@ -213,8 +215,21 @@ class Handle:
     def set_plan(self, n: int, rows: int = 0, cols: int = 0, precision: int = abi.PREC_FP32_MIXED):
         _check(lib().shg_set_plan(self._h, n, precision, rows, cols))
 
+    def get_plan(self, n: int, precision: int = abi.PREC_FP32_MIXED) -> tuple[int, int]:
+        """(rows per row-pass workgroup, columns per column-pass workgroup)."""
+        r, c = C.c_int32(), C.c_int32()
+        _check(lib().shg_get_plan(self._h, n, precision, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
     def autotune(self, n: int, precision: int = abi.PREC_FP32_MIXED, batch: int = 1, nz: int = 64):
+        """Time every supported plan for n x n on a synthetic march of `batch`
+        instances and keep the fastest; returns the chosen plan."""
         _check(lib().shg_autotune(self._h, n, precision, batch, nz))
+        return self.get_plan(n, precision)
+
+    def set_graph_steps(self, steps: int):
+        """Replay the fused march as hipGraphs of `steps` z-steps (0: direct launches)."""
+        _check(lib().shg_set_graph_steps(self._h, steps))
 
 
 def save_matrix_complex(A: np.ndarray, name: str, out_dir: str = ".") -> None:

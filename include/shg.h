@@ -136,6 +136,11 @@ typedef struct shg_params {
     double heat_tol_K;       /* DIRECT refinement tolerance [K] */
 
     /* extensions without a reference counterpart */
+    double waist_y_um;       /* elliptical pump: waist along y (waist_um is the
+                                x waist); 0 = circular, the reference's beam.
+                                The variant commented out at
+                                src/headers/Efields.h:113-115, normalised to
+                                power_W with sqrt(wx wy) */
     const double* poling_phase; /* nullable; nz entries [rad], added to the
                                    QPM phase of slice s (pump -, SH +) */
 } shg_params;
@@ -258,15 +263,36 @@ SHG_API int shg_last_kernel_times(shg_handle* h, const char** names, double* ms,
  * stream; read the totals with shg_last_kernel_times.  Resets the totals. */
 SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
 
+/* Row-pass plan value of the wide form: n threads per row, one point each
+ * in the nonlinear step, max(1, 256/n) rows per workgroup -- for launches
+ * too small to fill the GPU with n/8 threads per row. */
+#define SHG_PLAN_ROWS_WIDE 16
+
 /* Launch plan of the fused passes for transform length n and precision
  * (SHG_PREC_FP32_MIXED, _FP32_STORAGE or _FP64_FUSED): rows per workgroup of
- * the row pass and columns per workgroup of the column pass (0 keeps the
- * current choice).  Used by the autotuner. */
+ * the row pass (1, 2, 4, 8 with n/8 threads per row holding 8 points each,
+ * or SHG_PLAN_ROWS_WIDE) and columns per workgroup of the column pass
+ * (2, 4, 8, 16); 0 keeps the current choice.  Replaces the reference's
+ * fixed 16x16 blocks (src/headers/Solver.h:119-135) and per-slice cuFFT
+ * plans (src/headers/Solver.h:159,183). */
 SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t rows_per_wg, int32_t cols_per_wg);
 
-/* Time the candidate plans for transform length n on a synthetic march of
- * `batch` instances and keep the fastest on the handle. */
+/* The plan the handle uses for (n, precision): tuned, set, or built in. */
+SHG_API int shg_get_plan(shg_handle* h, int32_t n, int32_t precision, int32_t* rows_per_wg, int32_t* cols_per_wg);
+
+/* Autotuner: time every supported plan for transform length n on a
+ * synthetic non-THERMAL march of `batch` n x n instances with nz slices
+ * (HIP events around each launch) and keep the fastest row and column
+ * plans on the handle.  No reference counterpart. */
 SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int32_t nz);
+
+/* Graph replay of the fused march: `steps` > 0 captures the launches of
+ * that many consecutive z-steps (every launch group's row and column pass)
+ * once into a hipGraph, cached on the handle, and replays it along the
+ * march; 0 launches every kernel directly.  Runs with a per-slice trace or
+ * with profiling on always launch directly.  Replaces the reference's
+ * per-slice host loop with its synchronisations (src/headers/Solver.h:276-308). */
+SHG_API int shg_set_graph_steps(shg_handle* h, int32_t steps);
 
 #ifdef __cplusplus
 }

@@ -305,6 +305,22 @@ void orc_pump_init(const shg_params* p, double Tc, double* pump) {
     const double uX = 0.5 * g.nx, uY = 0.5 * g.ny;
     const double w2 = p->waist_um * p->waist_um;
     const double zR = ORC_PI * np * w2 / g.lp;
+    if (p->waist_y_um > 0) {
+        /* elliptical variant, commented out at src/headers/Efields.h:113-115:
+         * A = Ap0 / (sqrt(MX) sqrt(MY)) exp(-x^2/(wX^2 MX) - y^2/(wY^2 MY)),
+         * with Ap0 normalised by wX wY (the reference's line keeps wX^2) */
+        const double wy2 = p->waist_y_um * p->waist_y_um;
+        const double zRY = ORC_PI * np * wy2 / g.lp;
+        const double Ap0 = sqrt(4.0 * p->power_W / (ORC_EPS0 * ORC_C * ORC_PI * np * p->waist_um * p->waist_y_um));
+        const cplx MX = 1.0 - I * (p->focal_um / zR), MY = 1.0 - I * (p->focal_um / zRY);
+        const cplx amp = Ap0 / (csqrt(MX) * csqrt(MY));
+        for (int y = 0; y < g.ny; y++)
+            for (int x = 0; x < g.nx; x++) {
+                const double rx = (x - uX) * g.dx, ry = (y - uY) * g.dy;
+                A[(size_t)y * g.nx + x] = amp * cexp(-rx * rx / (w2 * MX) - ry * ry / (wy2 * MY));
+            }
+        return;
+    }
     const double Ap0 = sqrt(4.0 * p->power_W / (ORC_EPS0 * ORC_C * ORC_PI * np * w2));
     const cplx M = 1.0 - I * (p->focal_um / zR);
     for (int y = 0; y < g.ny; y++)

@@ -140,3 +140,7 @@ def test_null_arguments_are_einval_not_crashes():
     assert L.shg_run(None, None, None) == abi.SHG_EINVAL
     assert L.shg_run_batch(None, None, 0, None) == abi.SHG_EINVAL
     assert L.shg_handle_destroy(None) == abi.SHG_OK
+    assert L.shg_set_graph_steps(None, 4) == abi.SHG_EINVAL
+    assert L.shg_get_plan(None, 64, abi.PREC_FP32_MIXED, None, None) == abi.SHG_EINVAL
+    assert L.shg_autotune(None, 64, abi.PREC_FP32_MIXED, 1, 8) == abi.SHG_EINVAL
+    assert L.shg_set_plan(None, 64, abi.PREC_FP32_MIXED, 1, 4) == abi.SHG_EINVAL

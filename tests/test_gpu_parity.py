@@ -106,6 +106,35 @@ def test_walkoff_and_offcentre_focus(handle, form):
 
 
 @pytest.mark.gpu
+def test_elliptical_pump(handle, form):
+    """Astigmatic pump (waist_y_um set), off-centre focus, rectangular window."""
+    p = configs.spplt64()
+    p.nx, p.ny, p.nz = 128, 64, 30
+    p.lx_um, p.ly_um = 900.0, 600.0
+    p.waist_y_um = 1.6 * p.waist_um
+    p.focal_um = 0.3 * p.lz_um
+    p.power_W = 8.0
+    _check(p, handle, form)
+
+
+@pytest.mark.gpu
+def test_elliptical_pump_plain_and_thermal(handle):
+    p = configs.spplt64()
+    p.waist_y_um = 0.7 * p.waist_um
+    p.precision = abi.PREC_FP64_PLAIN
+    ro, o = O.run(p)
+    rg, g = handle.run(p)
+    assert rel(rg.eta, ro.eta) < 1e-11 and rel_l2(g["signal"], o["signal"]) < 1e-11
+    # waist_y_um == waist_um is the circular beam
+    q = p.copy()
+    q.waist_y_um = q.waist_um
+    r1, _ = handle.run(q)
+    q.waist_y_um = 0.0
+    r0, _ = handle.run(q)
+    assert rel(r1.eta, r0.eta) < 1e-12
+
+
+@pytest.mark.gpu
 def test_poling_phase(handle, form):
     p = configs.adjoint256(nz=60, n=64)
     _check(p, handle, form)

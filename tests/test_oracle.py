@@ -158,6 +158,57 @@ def test_free_space_gaussian_matches_analytic():
     assert r.P_pump_out_W == pytest.approx(r.P_pump_in_W, rel=1e-12)
 
 
+def _astigmatic_gauss(p, Tc, zrel):
+    """Analytic elliptical (astigmatic) Gaussian beam: each transverse axis is
+    an independent 1D Gaussian beam, amplitude 1/sqrt(M) per axis, with
+    M = 1 + i (z - f) / zR of its own waist; normalised to power_W."""
+    n = O.refractive_index(p.crystal, p.lambda_p_um, Tc)
+    wx, wy = p.waist_um, p.waist_y_um
+    Ap0 = math.sqrt(4 * p.power_W / (EPS0 * C_LIGHT * math.pi * n * wx * wy))
+    Mx = 1 + 1j * (zrel - p.focal_um) / (math.pi * n * wx ** 2 / p.lambda_p_um)
+    My = 1 + 1j * (zrel - p.focal_um) / (math.pi * n * wy ** 2 / p.lambda_p_um)
+    dx, dy = p.lx_um / (p.nx - 1), p.ly_um / (p.ny - 1)
+    x = (np.arange(p.nx) - p.nx / 2) * dx
+    y = (np.arange(p.ny) - p.ny / 2) * dy
+    return (Ap0 / (np.sqrt(Mx) * np.sqrt(My))) * np.exp(-x[None, :] ** 2 / (wx ** 2 * Mx)
+                                                         - y[:, None] ** 2 / (wy ** 2 * My))
+
+
+def test_elliptical_pump_is_the_astigmatic_gaussian():
+    """waist_y_um > 0: the pump is the separable astigmatic Gaussian (rank 1,
+    each axis with its own Rayleigh range), carries power_W, reduces to the
+    circular beam when the waists agree, and free-space diffraction carries
+    it to the analytic beam at the crystal end (a wrong axis, a missing
+    sqrt(M) or a circular M would each fail one of these)."""
+    p = configs.spplt64()
+    p.nx = p.ny = 128
+    p.lx_um = p.ly_um = 1000.0
+    p.waist_y_um = 1.7 * p.waist_um
+    p.focal_um = 0.3 * p.lz_um
+    A = O.pump_init(p, p.T_C)
+    assert np.abs(A - _astigmatic_gauss(p, p.T_C, 0.0)).max() < 1e-12 * np.abs(A).max()
+    sv = np.linalg.svd(A, compute_uv=False)
+    assert sv[1] < 1e-12 * sv[0]
+    n = O.refractive_index(0, 1.064, p.T_C)
+    assert O.power(p, A, n) == pytest.approx(p.power_W, rel=1e-6)
+    # |A|^2 falls to 1/e^2 at x = wx on the focus plane (f = 0)
+    q = p.copy()
+    q.focal_um = 0.0
+    B = O.pump_init(q, q.T_C)
+    assert B[q.ny // 2, q.nx // 2] == pytest.approx(math.sqrt(4 * q.power_W / (
+        EPS0 * C_LIGHT * math.pi * n * q.waist_um * q.waist_y_um)), rel=1e-14)
+    c = q.copy()
+    c.waist_y_um = c.waist_um
+    q.waist_y_um = 0.0
+    assert np.abs(O.pump_init(c, c.T_C) - O.pump_init(q, q.T_C)).max() < 1e-14 * np.abs(B).max()
+    # propagation through the crystal without coupling
+    p.deff_um_per_V = 0.0
+    p.nz = 30
+    r, o = O.run(p)
+    ref = _astigmatic_gauss(p, p.T_C, p.lz_um)
+    assert np.abs(o["pump"] - ref).max() < 1e-8 * np.abs(ref).max()
+
+
 def _plane_wave_run(p, steps, P0):
     P = np.full((p.ny, p.nx), P0, np.complex128)
     S = np.zeros_like(P)
