@@ -22,8 +22,12 @@ The JSON line also carries:
                   rocprofv3 counter summary (profiles/pmc_traffic.json).
   fp64_plain      the plain FP64 GPU form (one instance at a time) over
                   every instance: its throughput, and
-  eta_rel_err_vs_fp64  the eta error of the timed form against it.
+  eta_rel_err_vs_fp64  the eta error of the timed form against it (and
+                  P_sh_rel_err_vs_fp64_max, peak_dT_rel_err_vs_fp64: the
+                  SH power and peak temperature rise errors).
   other_fused_form     the other FP32-field form's value and eta error.
+  adjoint         forward + adjoint (shg_adjoint) wall time over one forward
+                  at 256x256 x 2000 slices.
   cpu_baseline    the FP64 oracle (OpenMP, all host threads) on a bounded
                   sample of the first instance.
 """
@@ -60,6 +64,28 @@ def rowpass_thermal_march_bytes(p) -> float:
     pts = p.nx * p.ny
     per = (p.nz + 1) * (32 + 24) + (nzh - 1) * 24 + p.nz * 8 - nzh * 4
     return float(per * pts)
+
+
+def adjoint_cost(h) -> dict:
+    """Cost of d eta / d(waist, focus, T, poling[2000]) by shg_adjoint
+    relative to one forward run: the 256x256, 2000-slice poled MgO:sPPLT
+    case (cushg_amd.configs.adjoint256), FP32 fields / FP64 arithmetic,
+    checkpoints every 45 slices; wall times of the second of two calls."""
+    from cushg_amd import configs
+    p = configs.adjoint256()
+    p.precision = abi.PREC_FP32_STORAGE
+    h.run(p, fields=False)
+    t = time.perf_counter()
+    h.run(p, fields=False)
+    tf = time.perf_counter() - t
+    h.adjoint(p, 45)
+    t = time.perf_counter()
+    h.adjoint(p, 45)
+    ta = time.perf_counter() - t
+    print(f"bench: adjoint {ta * 1e3:.1f} ms vs forward {tf * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    return {"workload": "256x256, 2000 slices, 50 mm MgO:sPPLT, seeded poling phases, checkpoints every 45",
+            "forward_ms": round(tf * 1e3, 2), "forward_plus_adjoint_ms": round(ta * 1e3, 2),
+            "ratio": round(ta / tf, 3), "target": 3.0}
 
 
 def workload(name: str):
@@ -197,6 +223,8 @@ def main():
                     help="fused form timed as the value: FP32 fields with FP64 arithmetic (storage) or "
                          "FP32 arithmetic (mixed); the other is reported beside it")
     ap.add_argument("--no-alt", action="store_true", help="skip the other fused form")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint cost line (256x256, 2000 slices, FP32 storage)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -280,14 +308,26 @@ def main():
                                "eta": [round(r.eta, 10) for r in r64[:4]]}
         extra["eta_rel_err_vs_fp64"] = {"max": max(errs), "mean": sum(errs) / len(errs), "bar": 1e-5,
                                         "instances": len(errs)}
+        # P_SH = eta * P carries eta's relative error; the peak temperature
+        # rise (THERMAL) against the FP64 form's, relative, over all instances
+        dts = [abs(a.peak_dT_K - b.peak_dT_K) / abs(b.peak_dT_K) for a, b in zip(res, r64) if b.peak_dT_K]
+        if dts:
+            extra["peak_dT_rel_err_vs_fp64"] = {"max": max(dts), "mean": sum(dts) / len(dts)}
+        pso = [abs(a.P_sh_W - b.P_sh_W) / abs(b.P_sh_W) for a, b in zip(res, r64) if b.P_sh_W]
+        extra["P_sh_rel_err_vs_fp64_max"] = max(pso) if pso else None
         extra["speedup_vs_fp64_plain"] = round(value / v64, 2)
         if alt is not None:
             ae = [abs(a.eta - b.eta) / abs(b.eta) for a, b in zip(ralt, r64)]
             alt["eta_rel_err_vs_fp64_max"] = max(ae)
+            adt = [abs(a.peak_dT_K - b.peak_dT_K) / abs(b.peak_dT_K) for a, b in zip(ralt, r64) if b.peak_dT_K]
+            if adt:
+                alt["peak_dT_rel_err_vs_fp64_max"] = max(adt)
             alt["speedup_vs_fp64_plain"] = round(alt["value"] / v64, 2)
         print(f"bench: fp64 plain {d64:.2f} s, eta rel err max {max(errs):.2e}", file=sys.stderr, flush=True)
     if alt is not None:
         extra["other_fused_form"] = alt
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint_cost(h)
 
     line = {
         "metric": METRIC,
