@@ -71,7 +71,7 @@ def adjoint_cost(h) -> dict:
     relative to one forward run: the 256x256, 2000-slice poled MgO:sPPLT
     case (cushg_amd.configs.adjoint256), FP32 fields / FP64 arithmetic,
     checkpoints every 45 slices; wall times of the second of two calls."""
-    from cushg_amd import configs
+    from cushg_amd import abi, configs
     p = configs.adjoint256()
     p.precision = abi.PREC_FP32_STORAGE
     h.run(p, fields=False)

@@ -445,8 +445,12 @@ static int direct_solve(shg_handle* h, const shg_params* p, const HeatCoef& c, H
         info.converged_in_one = true;
         return SHG_OK;
     }
-    // defect correction until the update itself is below tol (each step
-    // contracts the error by ~1e-6 with FP32 transforms, ~1e-14 with FP64)
+    // defect correction until the error left is below tol (each step
+    // contracts the error by ~1e-6 with FP32 transforms, ~1e-14 with FP64).
+    // The error after an update of size mc is about rho * mc; once this
+    // state's solves have measured rho, the loop stops when ten times that
+    // estimate is below tol, one correction before an update itself falls
+    // below tol (the criterion used until rho is known).
     const double tol = p->heat_tol_K > 0 ? p->heat_tol_K : 1e-8;
     double mc = INFINITY, prev = INFINITY;
     int it = 0;
@@ -458,9 +462,14 @@ static int direct_solve(shg_handle* h, const shg_params* p, const HeatCoef& c, H
             set_error("heat: non-finite correction");
             return SHG_ENAN;
         }
-        if (it > 0 && prev > 0 && std::isfinite(prev)) hs.rho = std::max(mc / prev, 1e-16);
+        if (it > 0 && prev > 0 && std::isfinite(prev)) {
+            hs.rho = std::max(mc / prev, 1e-16);
+            hs.rho_max = std::max(hs.rho_max, hs.rho);
+        }
         prev = mc;
         if (mc < tol) break;
+        const double est = 10.0 * hs.rho_max;
+        if (hs.rho_max > 0 && est < 0.1 && est * mc < tol) break;
         if (it >= 40) {
             set_error("heat: defect correction did not reach %g K (last update %g K)", tol, mc);
             return SHG_ENOCONV;

@@ -25,6 +25,7 @@ struct HeatState {
     int nblocks = 0;
     void* solver = nullptr;    // direct-solver tables (opaque)
     double rho = 0.0;          // observed contraction of one defect-correction step
+    double rho_max = 0.0;      // largest contraction seen on this state's solves (0: none yet)
     bool converged_in_one = false;
 };
 
