@@ -113,7 +113,7 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     K = std::max(1, std::min(K, nsl));
     const int ncp = (nsl + K - 1) / K;
     const int nbr = adj_row_workgroups(m.nx, m.ny), nbc = adj_col_workgroups(m.nx), nbp = adj_pump_workgroups();
-    A.rows = plan_rows(h, m.nx, A.arith);
+    A.rows = plan_rows(h, m.nx, A.arith, m.ny);
     A.cols = plan_cols(h, m.ny, A.arith, m.nx);
 
     // segment k covers slices [kK, min(kK + K, nsl)); its recomputation

@@ -11,6 +11,14 @@
 
 #include "../../include/shg.h"
 
+// Floating-point contraction only within one source expression: a*b + c
+// written as one expression becomes one FMA in every kernel that inlines it,
+// and nothing else is fused.  The backend's context-dependent fusion (the
+// default for HIP) would let two kernels running the same arithmetic -- the
+// row pass and its wide form, say -- round differently.  Multiply-adds
+// spread over helper calls are written with fma() explicitly (axpy).
+#pragma clang fp contract(on)
+
 namespace shg {
 
 // src/headers/Crystal.h:19-21 (um, ps, W, V)
@@ -42,6 +50,13 @@ template <typename C> __host__ __device__ __forceinline__ C conjc(C a) { a.y = -
 template <typename C> __host__ __device__ __forceinline__ C mul_i(C a) { C r; r.x = -a.y; r.y = a.x; return r; }
 template <typename C> __host__ __device__ __forceinline__ C mul_mi(C a) { C r; r.x = a.y; r.y = -a.x; return r; }
 template <typename C> __host__ __device__ __forceinline__ decltype(C{}.x) abs2(C a) { return a.x * a.x + a.y * a.y; }
+// y + a x, one FMA per component
+__host__ __device__ __forceinline__ double2 axpy(double a, double2 x, double2 y) {
+    return make_double2(fma(a, x.x, y.x), fma(a, x.y, y.y));
+}
+__host__ __device__ __forceinline__ float2 axpy(float a, float2 x, float2 y) {
+    return make_float2(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y));
+}
 
 // Extraordinary index n(lambda, T).  MgO:sPPLT per src/headers/Crystal.h:54-70
 // (Bruner et al.); MgO:PPLN per Gayer et al., Appl. Phys. B 91, 343 (2008).

@@ -85,7 +85,7 @@ template <typename C, bool INV> __device__ __forceinline__ void dft3(C* x) {
     using R = decltype(C{}.x);
     const R c = R(-0.5);
     C t1 = x[1] + x[2];
-    C t2 = x[0] + c * t1;
+    C t2 = axpy(c, t1, x[0]);
     const C d = x[1] - x[2];
     C t3 = mk<C>(mulk<R>(d.x, 0.86602540378443864676), mulk<R>(d.y, 0.86602540378443864676));
     // forward: y1 = t2 - i t3, y2 = t2 + i t3

@@ -71,7 +71,8 @@ hipError_t launch_colpass(int ny, Arith a, int cols_pw, dim3 grid, hipStream_t s
 constexpr int kRowsWide = SHG_PLAN_ROWS_WIDE;
 int rowpass_rows_per_wg(int nx, int rows);  // rows one workgroup covers under a plan value
 int rowpass_rows_default(int nx, Arith a);
-int colpass_cols_auto(int ny, Arith a, long lines);  // by launch size
+int rowpass_rows_auto(int nx, Arith a, long lines);  // by launch size
+int colpass_cols_auto(int ny, Arith a, long lines);
 int colpass_cols_default(int ny, Arith a);
 bool colpass_cols_supported(int ny, int cols);
 bool rowpass_rows_supported(int nx, int rows);

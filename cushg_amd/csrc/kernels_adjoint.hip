@@ -61,11 +61,11 @@ __device__ __forceinline__ void rk4_adj(double2& lP, double2& lS, double2 P, dou
                                         double (&acc)[6]) {
     double2 k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
     dadz(P, S, e[0], kp, ks, k.aph, k.ash, bfac, k1p, k1s);
-    const double2 u2p = P + k.c1 * k1p, u2s = S + k.c1 * k1s;
+    const double2 u2p = axpy(k.c1, k1p, P), u2s = axpy(k.c1, k1s, S);
     dadz(u2p, u2s, e[1], kp, ks, k.aph, k.ash, bfac, k2p, k2s);
-    const double2 u3p = P + k.c2 * k2p, u3s = S + k.c2 * k2s;
+    const double2 u3p = axpy(k.c2, k2p, P), u3s = axpy(k.c2, k2s, S);
     dadz(u3p, u3s, e[1], kp, ks, k.aph, k.ash, bfac, k3p, k3s);
-    const double2 u4p = P + k.c3 * k3p, u4s = S + k.c3 * k3s;
+    const double2 u4p = axpy(k.c3, k3p, P), u4s = axpy(k.c3, k3s, S);
     (void)k4p;
     (void)k4s;
     double2 aP = lP, aS = lS;
@@ -78,20 +78,20 @@ __device__ __forceinline__ void rk4_adj(double2& lP, double2& lS, double2 P, dou
     dadz_adj(u4p, u4s, e[2], kp, ks, k.aph, k.ash, bfac, a4p, a4s, gP, gS, ge, acc[3], acc[4], acc[5]);
     aP = aP + gP;
     aS = aS + gS;
-    a3p = a3p + k.c3 * gP;
-    a3s = a3s + k.c3 * gS;
+    a3p = axpy(k.c3, gP, a3p);
+    a3s = axpy(k.c3, gS, a3s);
     const double2 ge2 = ge;
     dadz_adj(u3p, u3s, e[1], kp, ks, k.aph, k.ash, bfac, a3p, a3s, gP, gS, ge, acc[3], acc[4], acc[5]);
     aP = aP + gP;
     aS = aS + gS;
-    a2p = a2p + k.c2 * gP;
-    a2s = a2s + k.c2 * gS;
+    a2p = axpy(k.c2, gP, a2p);
+    a2s = axpy(k.c2, gS, a2s);
     ge1 = ge1 + ge;
     dadz_adj(u2p, u2s, e[1], kp, ks, k.aph, k.ash, bfac, a2p, a2s, gP, gS, ge, acc[3], acc[4], acc[5]);
     aP = aP + gP;
     aS = aS + gS;
-    a1p = a1p + k.c1 * gP;
-    a1s = a1s + k.c1 * gS;
+    a1p = axpy(k.c1, gP, a1p);
+    a1s = axpy(k.c1, gS, a1s);
     ge1 = ge1 + ge;
     dadz_adj(P, S, e[0], kp, ks, k.aph, k.ash, bfac, a1p, a1s, gP, gS, ge, acc[3], acc[4], acc[5]);
     aP = aP + gP;

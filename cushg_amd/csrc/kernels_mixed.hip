@@ -15,23 +15,28 @@ hipError_t colpass_f64(int, int, dim3, hipStream_t, const MixedInst*, Step, cons
 
 // Built-in tile plans (measured on MI355X: rows per workgroup of the row
 // pass, columns per workgroup of the column pass); shg_autotune refines them.
-// Rows up to 256 points take the wide form: measured faster for one
-// instance and for batches of 16 (256 x 256 THERMAL FP64 arithmetic: 40 us
-// against 58 us per launch); 384 and 512 keep NX/8 threads per row, which
-// is faster once a batch fills the chip (512 x 512 x 16: 108 us against
-// 160 us).  The row plan depends on the row length only, never on the
-// batch, so an instance gives the same bits alone and in any batch.
 int rowpass_rows_default(int nx, Arith a) {
-    (void)a;
+    const bool fp64 = a != Arith::F32;
     switch (nx) {
-        case 64:
-        case 128:
-        case 192:
-        case 256: return kRowsWide;
+        case 64: return fp64 ? 4 : 8;
+        case 128: return fp64 ? 2 : 4;
+        case 192: return 2;
+        case 256: return fp64 ? 1 : 2;
         case 384: return 1;
         case 512: return 1;
     }
     return 0;
+}
+// A launch whose NX/8-thread rows make fewer than 4096 waves (16 per CU)
+// leaves the chip underfilled and takes the wide form (8x the threads, one
+// point each in the nonlinear step): measured 16 us against 24 us for one
+// 512 x 512 THERMAL instance, 40 us against 58 us for 16 x 256 x 256; a
+// full 16 x 512 x 512 batch keeps NX/8 threads per row (108 us against
+// 160 us).  Both forms give the same bits (FMAs are contracted per source
+// expression only), so the choice may follow the launch size.
+int rowpass_rows_auto(int nx, Arith a, long lines) {
+    if (lines > 0 && lines * (nx / 8) < 4096L * 64) return kRowsWide;
+    return rowpass_rows_default(nx, a);
 }
 bool rowpass_rows_supported(int nx, int rows) {
     if (rows == kRowsWide) return nx >= 64 && nx <= 512;

@@ -44,9 +44,9 @@ const char* colpass_name(int ny, Arith a) {
     return s.c_str();
 }
 
-int plan_rows(shg_handle* h, int nx, Arith a) {
+int plan_rows(shg_handle* h, int nx, Arith a, long lines) {
     auto it = h->rows_plan.find({nx, (int)a});
-    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_default(nx, a);
+    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_auto(nx, a, lines);
 }
 int plan_cols(shg_handle* h, int ny, Arith a, long lines) {
     auto it = h->cols_plan.find({ny, (int)a});
@@ -242,8 +242,8 @@ static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G) {
         int maxn = 0;
         for (auto* m : gr.m) maxn = std::max(maxn, row ? m->ny : m->nx);
         long lines = 0;
-        for (auto* m : gr.m) lines += m->nx;
-        const int plan = row ? plan_rows(h, gr.key_n, gr.arith) : plan_cols(h, gr.key_n, gr.arith, lines);
+        for (auto* m : gr.m) lines += row ? m->ny : m->nx;
+        const int plan = row ? plan_rows(h, gr.key_n, gr.arith, lines) : plan_cols(h, gr.key_n, gr.arith, lines);
         const int per = row ? rowpass_rows_per_wg(gr.key_n, plan) : plan;
         const dim3 grid((maxn + per - 1) / per, row ? 1 : 2, (unsigned)gr.m.size());
         ls.push_back({row, &gr, plan, grid});
@@ -346,13 +346,15 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         for (auto& kv : rowg) {
             Group& gr = kv.second;
             int cnt = 0, maxny = 0;
+            long lines = 0;
             for (auto* m : gr.m) {
                 if (m->nz < g) break;
                 cnt++;
                 maxny = std::max(maxny, m->ny);
+                lines += m->ny;
             }
             if (!cnt) continue;
-            const int rows = plan_rows(h, gr.key_n, gr.arith);
+            const int rows = plan_rows(h, gr.key_n, gr.arith, lines);
             const int per = rowpass_rows_per_wg(gr.key_n, rows);
             dim3 grid((maxny + per - 1) / per, 1, cnt);
             ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith));

@@ -42,10 +42,10 @@ int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std
               SliceTrace* trace);
 const char* rowpass_name(int nx, bool th, Arith a);
 const char* colpass_name(int ny, Arith a);
-// plans: the handle's set or tuned plan, else the built-in one (for the
-// column pass, for a launch covering `lines` columns in total; 0: the
+// plans: the handle's set or tuned plan, else the built-in one for a launch
+// covering `lines` rows (row pass) or columns (column pass) in total (0: the
 // static default)
-int plan_rows(shg_handle* h, int nx, Arith a);
+int plan_rows(shg_handle* h, int nx, Arith a, long lines = 0);
 int plan_cols(shg_handle* h, int ny, Arith a, long lines = 0);
 // sum of |A|^2 into a device slot, for either storage type
 int field_sum_sq(shg_handle* h, Arith a, const void* A, long n, double* slot);

@@ -120,13 +120,21 @@ template <typename R> struct RkConst {
 // A + c*k with c = (c1, c2, c3), z offsets e[0], e[1] (stages 2 and 3), e[2].
 template <typename C, typename R = re_t<C>>
 __device__ __forceinline__ void rk4_point(C& P, C& S, const C (&e)[3], R kp, R ks, R bfac, const RkConst<R>& k) {
-    C k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
-    dadz(P, S, e[0], kp, ks, k.aph, k.ash, bfac, k1p, k1s);
-    dadz(P + k.c1 * k1p, S + k.c1 * k1s, e[1], kp, ks, k.aph, k.ash, bfac, k2p, k2s);
-    dadz(P + k.c2 * k2p, S + k.c2 * k2s, e[1], kp, ks, k.aph, k.ash, bfac, k3p, k3s);
-    dadz(P + k.c3 * k3p, S + k.c3 * k3s, e[2], kp, ks, k.aph, k.ash, bfac, k4p, k4s);
-    P = P + k.w * (k1p + R(2) * k2p + R(2) * k3p + k4p);
-    S = S + k.w * (k1s + R(2) * k2s + R(2) * k3s + k4s);
+    // A += w (k1 + 2 k2 + 2 k3 + k4), the slope sum accumulated as each
+    // stage ends so that only one stage's slopes are live at a time
+    C kp_, ks_, sp, ss;
+    dadz(P, S, e[0], kp, ks, k.aph, k.ash, bfac, kp_, ks_);
+    sp = kp_;
+    ss = ks_;
+    dadz(axpy(k.c1, kp_, P), axpy(k.c1, ks_, S), e[1], kp, ks, k.aph, k.ash, bfac, kp_, ks_);
+    sp = axpy(R(2), kp_, sp);
+    ss = axpy(R(2), ks_, ss);
+    dadz(axpy(k.c2, kp_, P), axpy(k.c2, ks_, S), e[1], kp, ks, k.aph, k.ash, bfac, kp_, ks_);
+    sp = axpy(R(2), kp_, sp);
+    ss = axpy(R(2), ks_, ss);
+    dadz(axpy(k.c3, kp_, P), axpy(k.c3, ks_, S), e[2], kp, ks, k.aph, k.ash, bfac, kp_, ks_);
+    P = axpy(k.w, sp + kp_, P);
+    S = axpy(k.w, ss + ks_, S);
 }
 
 __device__ __forceinline__ int heat_slice(int s, int nz, int nzh) {
@@ -448,9 +456,13 @@ __global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const Mix
 // Column pass of step g: column FFT -> * Prop -> column IFFT on COLS
 // adjacent columns of one field (blockIdx.y: 0 pump, 1 SH).
 // grid (ceil(max_nx/COLS), 2, n_instances), block (COLS, NY/8)
+// FP64 arithmetic: keep 4 waves per SIMD (<= 128 VGPRs), the most the
+// LDS footprint of the FP64 exchange allows anyway
+template <typename C> struct ColWaves { static constexpr int W = sizeof(C) == 16 ? 4 : 1; };
+
 template <int NY, int COLS, typename S, typename C>
-__global__ __launch_bounds__(NY / 8 * COLS) void k_colpass(const MixedInst* __restrict__ insts, Step gs,
-                                                           const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NY / 8 * COLS) __attribute__((amdgpu_waves_per_eu(ColWaves<C>::W)))
+void k_colpass(const MixedInst* __restrict__ insts, Step gs, const double2* __restrict__ tw) {
     const int g = gs.get();
     using R = re_t<C>;
     constexpr int TPC = NY / 8;

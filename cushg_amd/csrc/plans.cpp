@@ -72,8 +72,9 @@ int shg_get_plan(shg_handle* h, int32_t n, int32_t precision, int32_t* rows, int
         set_error("shg_get_plan: n=%d precision=%d not a fused-pass plan", n, precision);
         return SHG_EINVAL;
     }
-    *rows = plan_rows(h, n, arith_of(precision));
-    *cols = plan_cols(h, n, arith_of(precision));
+    // built-in plans as for one n x n instance
+    *rows = plan_rows(h, n, arith_of(precision), n);
+    *cols = plan_cols(h, n, arith_of(precision), n);
     return SHG_OK;
 }
 

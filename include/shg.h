@@ -277,7 +277,9 @@ SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
  * plans (src/headers/Solver.h:159,183). */
 SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t rows_per_wg, int32_t cols_per_wg);
 
-/* The plan the handle uses for (n, precision): tuned, set, or built in. */
+/* The plan the handle uses for (n, precision): tuned or set, else the
+ * built-in plan for one n x n instance (built-in plans follow the launch
+ * size: every plan gives the same results bit for bit). */
 SHG_API int shg_get_plan(shg_handle* h, int32_t n, int32_t precision, int32_t* rows_per_wg, int32_t* cols_per_wg);
 
 /* Autotuner: time every supported plan for transform length n on a

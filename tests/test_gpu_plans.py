@@ -1,17 +1,13 @@
 """Launch plans, the autotuner and graph replay of the fused march.
 
-A plan (rows per row-pass workgroup, columns per column-pass workgroup)
-only regroups independent rows / columns, and graph replay only changes
-how the same launches are issued, so every such plan and every graph chunk
-must give bit-identical fields and eta to the direct launches of the
-built-in plan -- for uneven batches (different grids and slice counts),
-THERMAL runs and every fused form.
-
-The wide row-pass form is a different kernel (one point per thread) with
-the same operations; with FP32 arithmetic it is bit-identical too, with
-FP64 arithmetic the compiler's FMA contraction may differ by an ulp, so
-it is held to FP64 round-off (FP64 fields: eta within 1e-13) or, where an
-ulp can flip a stochastically rounded FP32 store, to 1e-7 (FP32 fields).
+A plan (rows per row-pass workgroup or the wide one-point-per-thread form,
+columns per column-pass workgroup) only regroups independent rows /
+columns, and graph replay only changes how the same launches are issued, so
+every plan and every graph chunk must give bit-identical fields and eta to
+the direct launches of the built-in plan -- for uneven batches (different
+grids and slice counts), THERMAL runs and every fused form.  (The library
+contracts multiply-adds into FMAs per source expression only, so the wide
+and the standard row pass round alike.)
 """
 import numpy as np
 import pytest
@@ -54,21 +50,6 @@ def same(a, b):
         assert np.array_equal(x["pump"], y["pump"]) and np.array_equal(x["signal"], y["signal"])
 
 
-WIDE_TOL = {abi.PREC_FP32_MIXED: 0.0, abi.PREC_FP32_STORAGE: 1e-7, abi.PREC_FP64_FUSED: 1e-13}
-
-
-def close(a, b, tol):
-    if tol == 0.0:
-        return same(a, b)
-    ra, oa = a
-    rb, ob = b
-    for x, y in zip(ra, rb):
-        assert abs(x.eta - y.eta) <= tol * abs(y.eta) and x.steps_done == y.steps_done
-    for x, y in zip(oa, ob):
-        for f in ("pump", "signal"):
-            assert np.linalg.norm(x[f] - y[f]) <= 10 * tol * np.linalg.norm(y[f])
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", FUSED)
 @pytest.mark.parametrize("thermal", [False, True], ids=["optics", "thermal"])
@@ -91,7 +72,7 @@ def test_every_plan_gives_identical_results(precision, thermal):
         for n in (64, 128, 192):
             h.set_plan(n, 1, 8, precision)
         ref = run_all(h, ps)
-        close(default, ref, WIDE_TOL[precision])
+        same(default, ref)
         for rows, cols in ((1, 2), (8, 4), (2, 16), (4, 8)):
             for n in (64, 128, 192):
                 h.set_plan(n, rows, cols, precision)
@@ -114,7 +95,7 @@ def test_autotune_picks_a_supported_plan_and_keeps_results():
             assert rows in (1, 2, 4, 8, abi.PLAN_ROWS_WIDE) and cols in (2, 4, 8, 16)
             assert h.get_plan(n, prec) == (rows, cols)
             r, out = h.run(p)
-            close(([r], [out]), ([ref[0]], [ref[1]]), WIDE_TOL[prec] if rows == abi.PLAN_ROWS_WIDE else 0.0)
+            same(([r], [out]), ([ref[0]], [ref[1]]))
         with pytest.raises(S.ShgError):
             h.autotune(100, abi.PREC_FP32_MIXED)
         with pytest.raises(S.ShgError):
