@@ -200,8 +200,14 @@ def cpu_baseline(ps, budget_s: float):
     fixed = max(d1 - 16 * per, 0.0)
     nz = int(max(16, min(p.nz, (budget_s - fixed) / per)))
     q, r, dt = timed(nz)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(q.nx * q.ny * r.steps_done / dt, 1), "unit": UNIT,
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "cpu_model": model, "host_threads": os.cpu_count(),
             "sample": f"first instance, {q.nx}x{q.ny}, one thermo-optic pass of {r.steps_done} z-steps"
                       + (f" with a {q.nx}x{q.ny}x{q.nz_heat} heat solve" if q.thermal else "")
                       + f", {dt:.1f} s"}

@@ -184,6 +184,12 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     SHG_HIP(hipMemsetAsync(partR, 0, sizeof(double) * (size_t)nz * nbr * 12, h->stream));
     SHG_HIP(hipMemsetAsync(partC, 0, sizeof(double) * (size_t)nz * nbc * 2, h->stream));
     hipStream_t s1 = h->stream, s2 = h->stream2;
+    // whatever path leaves this function, the second stream has finished
+    // its work before the arena it reads is handed to another call
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{s2};
     hipEvent_t ev_fwd = h->ev[0], ev_rec[2] = {h->ev[1], h->ev[2]}, ev_adj[2] = {h->ev[3], h->ev[4]};
 
     // 1. forward march with checkpoints (state before slices 0, K, 2K, ...)
