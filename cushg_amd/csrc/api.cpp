@@ -243,6 +243,14 @@ int shg_handle_destroy(shg_handle* h) {
     if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
     heat_cache_free(h);
     for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto s : h->lanes)
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+    for (auto e : h->join_ev)
+        if (e) (void)hipEventDestroy(e);
     if (h->step_counter) (void)hipFree(h->step_counter);
     for (auto ev : h->ev)
         if (ev) (void)hipEventDestroy(ev);

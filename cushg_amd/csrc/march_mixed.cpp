@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -219,6 +220,50 @@ struct Group {
 using RowGroups = std::map<std::tuple<int, bool, int>, Group>;
 using ColGroups = std::map<std::pair<int, int>, Group>;
 
+// Launch groups that share no instance, directly or through a chain of
+// groups, are independent: each such component runs on its own stream (up
+// to four: the handle's two and two more created on first use), forked
+// from and joined back to the handle's stream.
+struct Lanes {
+    hipStream_t streams[4] = {};
+    std::map<const Group*, int> lane;
+    int n = 1;
+};
+static int march_lanes(shg_handle* h, RowGroups& rowg, ColGroups& colg, bool single, Lanes& L) {
+    std::vector<const Group*> gs;
+    std::map<const Group*, int> idx;
+    for (auto& kv : rowg) idx[&kv.second] = (int)gs.size(), gs.push_back(&kv.second);
+    for (auto& kv : colg) idx[&kv.second] = (int)gs.size(), gs.push_back(&kv.second);
+    std::vector<int> par(gs.size());
+    for (size_t i = 0; i < par.size(); i++) par[i] = (int)i;
+    std::function<int(int)> find = [&](int i) { return par[i] == i ? i : par[i] = find(par[i]); };
+    for (auto& kv : rowg)
+        for (auto* m : kv.second.m) {
+            const int a = find(idx[&kv.second]);
+            const int b = find(idx[&colg.at({m->ny, (int)m->arith})]);
+            par[a] = b;
+        }
+    std::map<int, int> comp;
+    for (size_t i = 0; i < gs.size(); i++) {
+        const int r = find((int)i);
+        if (!comp.count(r)) comp[r] = (int)comp.size();
+    }
+    L.n = single ? 1 : std::min<int>((int)comp.size(), 4);
+    for (size_t i = 0; i < gs.size(); i++) L.lane[gs[i]] = comp[find((int)i)] % L.n;
+    L.streams[0] = h->stream;
+    L.streams[1] = h->stream2;
+    if (L.n > 1) {
+        for (int l = 2; l < L.n; l++) {
+            if (!h->lanes[l - 2]) SHG_HIP(hipStreamCreateWithFlags(&h->lanes[l - 2], hipStreamNonBlocking));
+            L.streams[l] = h->lanes[l - 2];
+        }
+        if (!h->fork_ev) SHG_HIP(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+        for (auto& e : h->join_ev)
+            if (!e) SHG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    return SHG_OK;
+}
+
 // Graph replay of the march: the launches of `chunk` consecutive steps
 // (every group's row pass, then every group's column pass, per step) and
 // a counter advance are captured once into a graph whose kernels read the
@@ -230,14 +275,17 @@ using ColGroups = std::map<std::pair<int, int>, Group>;
 static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G) {
     if (!h->step_counter) SHG_HIP(hipMalloc(&h->step_counter, 256));
     const int C = h->graph_chunk;
+    Lanes L;
+    if (int rc = march_lanes(h, rowg, colg, false, L)) return rc;
     struct Launch {
         bool row;
         const Group* gr;
         int plan;
         dim3 grid;
+        int lane;
     };
     std::vector<Launch> ls;
-    std::vector<uint64_t> key{(uint64_t)C, (uint64_t)(uintptr_t)h->step_counter};
+    std::vector<uint64_t> key{(uint64_t)C, (uint64_t)(uintptr_t)h->step_counter, (uint64_t)L.n};
     auto add = [&](bool row, const Group& gr) {
         int maxn = 0;
         for (auto* m : gr.m) maxn = std::max(maxn, row ? m->ny : m->nx);
@@ -246,10 +294,11 @@ static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G) {
         const int plan = row ? plan_rows(h, gr.key_n, gr.arith, lines) : plan_cols(h, gr.key_n, gr.arith, lines);
         const int per = row ? rowpass_rows_per_wg(gr.key_n, plan) : plan;
         const dim3 grid((maxn + per - 1) / per, row ? 1 : 2, (unsigned)gr.m.size());
-        ls.push_back({row, &gr, plan, grid});
+        const int ln = L.lane.at(&gr);
+        ls.push_back({row, &gr, plan, grid, ln});
         for (uint64_t v : {(uint64_t)row, (uint64_t)gr.key_n, (uint64_t)gr.thermal, (uint64_t)gr.arith,
                            (uint64_t)plan, (uint64_t)grid.x, (uint64_t)grid.z, (uint64_t)(uintptr_t)gr.d,
-                           (uint64_t)(uintptr_t)gr.tw})
+                           (uint64_t)(uintptr_t)gr.tw, (uint64_t)ln})
             key.push_back(v);
     };
     for (auto& kv : rowg) add(true, kv.second);
@@ -263,17 +312,28 @@ static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G) {
             for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
             h->graphs.clear();
         }
+        // independent groups become parallel branches of the graph (fork
+        // from and join into the capturing stream); the counter advance
+        // follows every branch
         SHG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
+        if (L.n > 1) {
+            e = hipEventRecord(h->fork_ev, h->stream);
+            for (int l = 1; l < L.n && e == hipSuccess; l++) e = hipStreamWaitEvent(L.streams[l], h->fork_ev, 0);
+        }
         for (int c = 0; c < C && e == hipSuccess; c++)
             for (const Launch& l : ls) {
                 const Step st(h->step_counter, c);
-                e = l.row ? launch_rowpass(l.gr->key_n, l.gr->thermal, l.gr->arith, l.plan, l.grid, h->stream,
-                                           l.gr->d, st, l.gr->tw)
-                          : launch_colpass(l.gr->key_n, l.gr->arith, l.plan, l.grid, h->stream, l.gr->d, st,
-                                           l.gr->tw);
+                hipStream_t s = L.streams[l.lane];
+                e = l.row ? launch_rowpass(l.gr->key_n, l.gr->thermal, l.gr->arith, l.plan, l.grid, s, l.gr->d, st,
+                                           l.gr->tw)
+                          : launch_colpass(l.gr->key_n, l.gr->arith, l.plan, l.grid, s, l.gr->d, st, l.gr->tw);
                 if (e != hipSuccess) break;
             }
+        for (int l = 1; l < L.n && e == hipSuccess; l++) {
+            e = hipEventRecord(h->join_ev[l - 1], L.streams[l]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->join_ev[l - 1], 0);
+        }
         if (e == hipSuccess) e = launch_step_advance(h->stream, h->step_counter, C);
         hipGraph_t graph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
@@ -342,9 +402,19 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         for (auto* m : act) m->steps += m->nz;
         return SHG_OK;
     }
+    Lanes L;
+    if (int rc = march_lanes(h, rowg, colg, traced || h->profiling, L)) return rc;
+    hipStream_t* streams = L.streams;
+    auto& lane = L.lane;
+    const int nlanes = L.n;
+    if (nlanes > 1) {
+        SHG_HIP(hipEventRecord(h->fork_ev, h->stream));
+        for (int l = 1; l < nlanes; l++) SHG_HIP(hipStreamWaitEvent(streams[l], h->fork_ev, 0));
+    }
     for (int g = 0; g <= G; g++) {
         for (auto& kv : rowg) {
             Group& gr = kv.second;
+            hipStream_t st = streams[lane[&gr]];
             int cnt = 0, maxny = 0;
             long lines = 0;
             for (auto* m : gr.m) {
@@ -357,8 +427,8 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             const int rows = plan_rows(h, gr.key_n, gr.arith, lines);
             const int per = rowpass_rows_per_wg(gr.key_n, rows);
             dim3 grid((maxny + per - 1) / per, 1, cnt);
-            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith));
-            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, h->stream, gr.d, g, gr.tw));
+            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith), st);
+            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, st, gr.d, g, gr.tw));
         }
         for (auto* m : act)
             if (m->trace && m->trace->wants(g, m->nz))
@@ -366,6 +436,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         if (g == G) break;
         for (auto& kv : colg) {
             Group& gr = kv.second;
+            hipStream_t st = streams[lane[&gr]];
             int cnt = 0, maxnx = 0;
             long lines = 0;
             for (auto* m : gr.m) {
@@ -377,9 +448,13 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             if (!cnt) continue;
             const int cols = plan_cols(h, gr.key_n, gr.arith, lines);
             dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
-            ProfScope ps(h, colpass_name(gr.key_n, gr.arith));
-            SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, h->stream, gr.d, g, gr.tw));
+            ProfScope ps(h, colpass_name(gr.key_n, gr.arith), st);
+            SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, st, gr.d, g, gr.tw));
         }
+    }
+    for (int l = 1; l < nlanes; l++) {
+        SHG_HIP(hipEventRecord(h->join_ev[l - 1], streams[l]));
+        SHG_HIP(hipStreamWaitEvent(h->stream, h->join_ev[l - 1], 0));
     }
     // descriptors' host copies must outlive the async copies
     SHG_HIP(hipStreamSynchronize(h->stream));

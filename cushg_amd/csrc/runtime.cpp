@@ -71,6 +71,8 @@ void prof_collect(shg_handle* h) {
     if (h->pending.empty()) return;
     (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+    for (auto s : h->lanes)
+        if (s) (void)hipStreamSynchronize(s);
     for (auto& pe : h->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {

@@ -78,6 +78,11 @@ struct shg_handle {
     // stream's (the adjoint's segment recomputation), and ordering events
     hipStream_t stream2 = nullptr;
     hipEvent_t ev[6] = {};
+    // two more streams (created on first use) so that independent launch
+    // groups of a batched march run concurrently on up to four streams,
+    // with a fork event and per-stream join events
+    hipStream_t lanes[2] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {};
     shg::Arena arena;
     std::map<int, double2*> tw64;
     void* blas = nullptr;         // rocBLAS handle (heat solver), created lazily
