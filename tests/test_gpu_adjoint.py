@@ -104,6 +104,29 @@ def test_adjoint_walkoff_loss_and_strang(handle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nz", [2, 3, 5])
+def test_adjoint_few_slices(handle, nz):
+    """Marches of one to four slices: one segment recorded by the forward
+    itself, the output boundary and the first slice in the same segment."""
+    p = small_case(abi.PREC_FP64_FUSED)
+    p.nz = nz
+    p.set_poling(np.linspace(0.05, -0.05, nz))
+    print(f"case adjoint 64x64x{nz}", flush=True)
+    q = p.copy()
+    q.precision = abi.PREC_FP64_PLAIN
+    for ck in (0, 1, 7):
+        r, g = handle.adjoint(p, ck)
+        for i, (name, h) in enumerate(STEPS.items()):
+            f = fd_scalar(oracle_eta, q, name, h)
+            assert abs(g[i] - f) <= 1e-6 * abs(f), (ck, name, g[i], f)
+        fdp = np.array([fd_poling(oracle_eta, q, s) for s in range(nz)])
+        # absolute floor: the central difference cannot resolve less than
+        # eta's FP64 round-off over the step, ~ eta * 1e-16 / 1e-4
+        assert np.abs(g[3:] - fdp).max() <= 1e-6 * np.abs(fdp).max() + 1e-10 * r.eta, ck
+        assert g[3 + nz - 1] == 0.0
+
+
+@pytest.mark.gpu
 def test_checkpoint_interval_does_not_change_the_gradient(handle):
     p = small_case(abi.PREC_FP64_FUSED)
     p.nz = 40
