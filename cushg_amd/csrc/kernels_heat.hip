@@ -6,7 +6,11 @@
 // x-y plane, x fastest), keeping T at z-1, z, z+1 in registers: every
 // voxel's temperature is read from memory once per sweep (plus its x/y
 // neighbours, which hit the caches), as in the reference's in-thread
-// z-loop (src/headers/Tfield.h:111).
+// z-loop (src/headers/Tfield.h:111).  The z extent is cut into chunks
+// (grid.z) so that a column's dependent loads are spread over several
+// threads: one chunk re-reads only the plane below it.
+#include <algorithm>
+
 #include "heat_kernels.hpp"
 
 namespace shg {
@@ -51,16 +55,27 @@ __device__ __forceinline__ double jacobi_vals(const HeatCoef& c, int x, int y, i
     return (c.AV * xn + c.BV * t.ym + c.CV * zn + Tinf * c.BiD) * c.GV;
 }
 
-// Walks the z column of a non-Dirichlet node (x, y), calling
-// f(z, index, F(T), F(T) - T).
+// z range [z0, z1) of the chunk blockIdx.z
+struct ZRange {
+    int z0, z1;
+};
+__device__ __forceinline__ ZRange zchunk(const HeatCoef& c) {
+    const int per = (c.nz + gridDim.z - 1) / gridDim.z;
+    const int z0 = blockIdx.z * per;
+    return {z0, min(c.nz, z0 + per)};
+}
+
+// Walks the z range [z0, z1) of the column of a non-Dirichlet node (x, y),
+// calling f(z, index, F(T), F(T) - T).
 template <typename Fn>
 __device__ __forceinline__ void jacobi_column(const HeatCoef& c, const double* __restrict__ T,
-                                              const float* __restrict__ Q, int x, int y, Fn&& f) {
+                                              const float* __restrict__ Q, int x, int y, ZRange zr, Fn&& f) {
     const size_t sy = (size_t)c.nx, sz = (size_t)c.nx * c.ny;
-    size_t i = (size_t)y * sy + x;
+    size_t i = (size_t)zr.z0 * sz + (size_t)y * sy + x;
     const bool hx0 = x > 0, hx1 = x < c.nx - 1, hy1 = y < c.ny - 1;
-    double cur = T[i], nxt = c.nz > 1 ? T[i + sz] : 0.0, prv = 0.0;
-    for (int z = 0; z < c.nz; z++, i += sz) {
+    if (zr.z0 >= zr.z1) return;
+    double cur = T[i], nxt = zr.z0 + 1 < c.nz ? T[i + sz] : 0.0, prv = zr.z0 > 0 ? T[i - sz] : 0.0;
+    for (int z = zr.z0; z < zr.z1; z++, i += sz) {
         Nb t;
         t.c = cur;
         t.zm = prv;
@@ -104,16 +119,17 @@ __global__ __launch_bounds__(256) void k_heat_sweep(HeatCoef c, const double* __
                                                     double* __restrict__ part_sq, int want_norm) {
     __shared__ double red[4];
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const ZRange zr = zchunk(c);
     double d2 = 0.0;
     if (x < c.nx && y < c.ny) {
         const bool fixed = y == 0 || (c.oven && (x == 0 || x == c.nx - 1 || y == c.ny - 1));
         if (fixed) {  // oven nodes are copied through
-            for (int z = 0; z < c.nz; z++) {
+            for (int z = zr.z0; z < zr.z1; z++) {
                 const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
                 Tf[i] = Ti[i];
             }
         } else {
-            jacobi_column(c, Ti, Q, x, y, [&](int, size_t i, double v, double d) {
+            jacobi_column(c, Ti, Q, x, y, zr, [&](int, size_t i, double v, double d) {
                 Tf[i] = v;
                 d2 += d * d;
             });
@@ -121,7 +137,8 @@ __global__ __launch_bounds__(256) void k_heat_sweep(HeatCoef c, const double* __
     }
     if (want_norm) {
         const double s = block_reduce(d2, false, red);
-        if (threadIdx.x == 0 && threadIdx.y == 0) part_sq[blockIdx.y * gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            part_sq[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -135,12 +152,13 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
                                                     double* __restrict__ part_sq) {
     __shared__ double red[4];
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const ZRange zr = zchunk(c);
     double d2 = 0.0;
     const int xu = x - c.x0, j = y - 1;
     if (xu >= 0 && xu < c.nxu && j >= 0 && j < c.nyu) {
         const double wx = a.wx[xu], wy = a.wy[j], kx = a.kx[xu], ky = a.ky[j];
         const size_t nyu = (size_t)c.nyu, nxu = (size_t)c.nxu;
-        jacobi_column(c, T, Q, x, y, [&](int z, size_t, double, double d) {
+        jacobi_column(c, T, Q, x, y, zr, [&](int z, size_t, double, double d) {
             d2 += d * d;
             const double wz = a.wz[z];
             const double aii = kx * wy * wz + wx * ky * wz + wx * wy * a.kz[z];
@@ -148,21 +166,19 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
         });
     }
     const double s = block_reduce(d2, false, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part_sq[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        part_sq[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
 }
 
-// spectral division by lambda_x + lambda_y + lambda_z
+// spectral division by lambda_x + lambda_y + lambda_z; grid (ceil(nxu/256), nyu, nz)
 template <typename R>
-__global__ void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx, const double* __restrict__ ly,
-                            const double* __restrict__ lz, int nxu, int nyu, int nz) {
-    const long n = (long)nxu * nyu * nz;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
-        const int x = (int)(i % nxu);
-        const long r = i / nxu;
-        const int j = (int)(r % nyu);
-        const int z = (int)(r / nyu);
-        v[i] = (R)((double)v[i] / (lx[x] + ly[j] + lz[z]));
-    }
+__global__ __launch_bounds__(256) void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx,
+                                                   const double* __restrict__ ly, const double* __restrict__ lz,
+                                                   int nxu, int nyu) {
+    const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, z = blockIdx.z;
+    if (x >= nxu) return;
+    const size_t i = ((size_t)z * nyu + j) * nxu + x;
+    v[i] = (R)((double)v[i] / (lx[x] + ly[j] + lz[z]));
 }
 
 // T += W^-1/2 u (the correction); partial max |correction|
@@ -171,19 +187,21 @@ __global__ __launch_bounds__(256) void k_fd_update(HeatCoef c, HeatAxes a, const
                                                    double* __restrict__ T, double* __restrict__ part_max) {
     __shared__ double red[4];
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const ZRange zr = zchunk(c);
     double m = 0.0;
     const int xu = x - c.x0, j = y - 1;
     if (xu >= 0 && xu < c.nxu && j >= 0 && j < c.nyu) {
         const double wxy = a.wx[xu] * a.wy[j];
         const size_t nyu = (size_t)c.nyu, nxu = (size_t)c.nxu;
-        for (int z = 0; z < c.nz; z++) {
+        for (int z = zr.z0; z < zr.z1; z++) {
             const double corr = (double)u[((size_t)z * nyu + j) * nxu + xu] * rsqrt(wxy * a.wz[z]);
             T[((size_t)z * c.ny + y) * c.nx + x] += corr;
             m = fmax(m, fabs(corr));
         }
     }
     const double s = block_reduce(m, true, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part_max[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        part_max[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part, int n, int op,
@@ -265,10 +283,15 @@ __global__ __launch_bounds__(256) void k_max(const double* __restrict__ a, long 
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
-static dim3 plane_grid(const HeatCoef& c) { return dim3((c.nx + 63) / 64, (c.ny + 3) / 4); }
+// x-y plane tiles of 64 x 4 columns, and z chunks of at least 48 slices (at
+// most 8 chunks)
+static dim3 plane_grid(const HeatCoef& c) {
+    const int zc = std::max(1, std::min(8, c.nz / 48));
+    return dim3((c.nx + 63) / 64, (c.ny + 3) / 4, zc);
+}
 int heat_grid_blocks(const HeatCoef& c) {
     const dim3 g = plane_grid(c);
-    return (int)(g.x * g.y);
+    return (int)(g.x * g.y * g.z);
 }
 
 hipError_t launch_heat_sweep(hipStream_t st, const HeatCoef& c, const double* Ti, double* Tf, const float* Q,
@@ -290,10 +313,8 @@ template hipError_t launch_heat_resid<double>(hipStream_t, const HeatCoef&, cons
 
 template <typename R>
 hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz) {
-    const long n = (long)nxu * nyu * nz;
-    long nb = (n + 255) / 256;
-    if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL((k_fd_divide<R>), dim3((unsigned)nb), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu, nyu, nz);
+    hipLaunchKernelGGL((k_fd_divide<R>), dim3((nxu + 255) / 256, nyu, nz), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu,
+                       nyu);
     return hipGetLastError();
 }
 template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int);
