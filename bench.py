@@ -229,6 +229,9 @@ def main():
                     help="fused form timed as the value: FP32 fields with FP64 arithmetic (storage) or "
                          "FP32 arithmetic (mixed); the other is reported beside it")
     ap.add_argument("--no-alt", action="store_true", help="skip the other fused form")
+    ap.add_argument("--autotune", action="store_true",
+                    help="before timing, run shg_autotune for every grid size of the workload (batch = the "
+                         "workload's instances of that size); the tuning time is not in the timed region")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost line (256x256, 2000 slices, FP32 storage)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -270,6 +273,13 @@ def main():
     print(f"bench: {args.workload} ({len(ps)} instances) warmup={args.warmup} steps={args.steps}",
           file=sys.stderr, flush=True)
     h = shg.Handle(0)
+    plans = {}
+    if args.autotune:
+        from collections import Counter
+        t = time.perf_counter()
+        for n, cnt in sorted(Counter(p.nx for p in ps).items()):
+            plans[n] = h.autotune(n, prec, batch=cnt, nz=64)
+        print(f"bench: autotuned {plans} in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         res = run_once(h, ps)
     t0 = time.perf_counter()
@@ -354,6 +364,7 @@ def main():
                          "reductions and heat residual in f64"),
         "data": "synthetic (Gaussian pump from parameters)",
         "config": {"workload": f"{args.workload}: {desc}", "instances": len(ps),
+                   "plans": {str(n): list(v) for n, v in plans.items()} if plans else "built-in",
                    "form": "FP32 batched, " + ("FP32 fields / FP64 arithmetic (SHG_PREC_FP32_STORAGE)"
                                                if prec == abi.PREC_FP32_STORAGE else
                                                "FP32 arithmetic (SHG_PREC_FP32_MIXED)"),

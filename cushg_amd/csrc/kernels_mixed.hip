@@ -13,29 +13,31 @@ hipError_t colpass_f32(int, int, dim3, hipStream_t, const MixedInst*, Step, cons
 hipError_t colpass_s32(int, int, dim3, hipStream_t, const MixedInst*, Step, const double2*);
 hipError_t colpass_f64(int, int, dim3, hipStream_t, const MixedInst*, Step, const double2*);
 
-// Built-in tile plans (measured on MI355X: rows per workgroup of the row
-// pass, columns per workgroup of the column pass); shg_autotune refines them.
+// Built-in tile plans: rows per workgroup of the row pass and columns per
+// workgroup of the column pass, from shg_autotune runs on MI355X over
+// batches of 1, 4 and 16 instances per size (they agree with the tuned
+// plans at most sizes; shg_autotune refines them for a given workload).
 int rowpass_rows_default(int nx, Arith a) {
     const bool fp64 = a != Arith::F32;
     switch (nx) {
         case 64: return fp64 ? 4 : 8;
         case 128: return fp64 ? 2 : 4;
         case 192: return 2;
-        case 256: return fp64 ? 1 : 2;
-        case 384: return 1;
-        case 512: return 1;
+        case 256: return 2;
+        case 384: return 4;
+        case 512: return fp64 ? 1 : 2;
     }
     return 0;
 }
-// A launch whose NX/8-thread rows make fewer than 4096 waves (16 per CU)
+// A launch whose NX/8-thread rows make fewer than 1024 waves (4 per CU)
 // leaves the chip underfilled and takes the wide form (8x the threads, one
-// point each in the nonlinear step): measured 16 us against 24 us for one
-// 512 x 512 THERMAL instance, 40 us against 58 us for 16 x 256 x 256; a
-// full 16 x 512 x 512 batch keeps NX/8 threads per row (108 us against
-// 160 us).  Both forms give the same bits (FMAs are contracted per source
-// expression only), so the choice may follow the launch size.
+// point each in the nonlinear step): one 512 x 512 THERMAL instance runs
+// at 16 us against 24 us per launch, while a 16 x 512 x 512 batch keeps
+// NX/8 threads per row (108 us against 160 us).  Both forms give the same
+// bits (FMAs are contracted per source expression only), so the choice may
+// follow the launch size.
 int rowpass_rows_auto(int nx, Arith a, long lines) {
-    if (lines > 0 && lines * (nx / 8) < 4096L * 64) return kRowsWide;
+    if (lines > 0 && lines * (nx / 8) < 1024L * 64) return kRowsWide;
     return rowpass_rows_default(nx, a);
 }
 bool rowpass_rows_supported(int nx, int rows) {
@@ -47,16 +49,23 @@ int rowpass_rows_per_wg(int nx, int rows) {
     return rows;
 }
 int colpass_cols_default(int ny, Arith a) {
-    (void)ny;
-    (void)a;
+    const bool fp64 = a != Arith::F32;
+    switch (ny) {
+        case 64: return 8;
+        case 128: return 4;
+        case 192: return 16;
+        case 256: return fp64 ? 16 : 8;
+        case 384: return fp64 ? 16 : 4;
+        case 512: return 8;
+    }
     return 8;
 }
 // Launch-size-aware column plan: the column pass narrows its tiles until
-// the launch has at least 512 workgroups (every column plan gives the same
+// the launch has at least 256 workgroups (every column plan gives the same
 // bits, so this may depend on the batch).
 int colpass_cols_auto(int ny, Arith a, long lines) {
     int c = colpass_cols_default(ny, a);
-    while (lines > 0 && c > 2 && lines * 2 / c < 512) c /= 2;
+    while (lines > 0 && c > 2 && lines * 2 / c < 256) c /= 2;
     return c;
 }
 bool colpass_cols_supported(int ny, int cols) {
