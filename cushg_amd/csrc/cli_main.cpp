@@ -8,7 +8,7 @@
 // time (src/cuSHG.cu:11-15, src/headers/Crystal.h:26-52,
 // src/headers/Solver.h:32); here they are options:
 //   --nx= --ny= --nz= --nz-heat= --lx= --ly= --lz= (um)
-//   --crystal=spplt|ppln  --thermal=0|1  --precision=fp32|fp64
+//   --crystal=spplt|ppln  --thermal=0|1  --precision=fp32|fp32-storage|fp64|fp64-fused
 //   --heat=jacobi|direct  --heat-bc=open|oven  --scheme=reference|strang  --max-outer=
 //   --deff= (um/V) --period= (um) --alpha-p= --alpha-s= (1/um) --beta-s= (um/W)
 //   --rho-p= --rho-s= (rad) --waist-y=UM (elliptical pump; default circular)
@@ -16,10 +16,27 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
 #include "../../include/shg.h"
+
+// value of a named choice: false (with a message) for an unknown name
+static bool pick(const char* v, std::initializer_list<const char*> names, std::initializer_list<int> values,
+                 const char* option, int32_t& out) {
+    auto n = names.begin();
+    for (int x : values) {
+        if (std::strcmp(v, *n++) == 0) {
+            out = x;
+            return true;
+        }
+    }
+    std::fprintf(stderr, "shg: %s=%s is not one of", option, v);
+    for (const char* nm : names) std::fprintf(stderr, " %s", nm);
+    std::fprintf(stderr, "\n");
+    return false;
+}
 
 static bool opt(const char* a, const char* key, const char** val) {
     const size_t n = std::strlen(key);
@@ -64,13 +81,19 @@ int main(int argc, char** argv) {
         } else if (opt(a, "--thermal", &v)) {
             p.thermal = std::atoi(v);
         } else if (opt(a, "--precision", &v)) {
-            p.precision = std::strcmp(v, "fp64") == 0 ? SHG_PREC_FP64_PLAIN : SHG_PREC_FP32_MIXED;
+            if (!pick(v, {"fp32", "fp32-storage", "fp64", "fp64-fused"},
+                      {SHG_PREC_FP32_MIXED, SHG_PREC_FP32_STORAGE, SHG_PREC_FP64_PLAIN, SHG_PREC_FP64_FUSED}, "--precision",
+                      p.precision))
+                return 2;
         } else if (opt(a, "--heat", &v)) {
-            p.heat_solver = std::strcmp(v, "direct") == 0 ? SHG_HEAT_DIRECT : SHG_HEAT_JACOBI_REFERENCE;
+            if (!pick(v, {"jacobi", "direct"}, {SHG_HEAT_JACOBI_REFERENCE, SHG_HEAT_DIRECT}, "--heat", p.heat_solver))
+                return 2;
         } else if (opt(a, "--heat-bc", &v)) {
-            p.heat_bc = std::strcmp(v, "oven") == 0 ? SHG_HEAT_BC_OVEN_SURROUNDED : SHG_HEAT_BC_OPEN_TOP;
+            if (!pick(v, {"open", "oven"}, {SHG_HEAT_BC_OPEN_TOP, SHG_HEAT_BC_OVEN_SURROUNDED}, "--heat-bc", p.heat_bc))
+                return 2;
         } else if (opt(a, "--scheme", &v)) {
-            p.scheme = std::strcmp(v, "strang") == 0 ? SHG_SCHEME_STRANG : SHG_SCHEME_REFERENCE;
+            if (!pick(v, {"reference", "strang"}, {SHG_SCHEME_REFERENCE, SHG_SCHEME_STRANG}, "--scheme", p.scheme))
+                return 2;
         } else if (opt(a, "--max-outer", &v)) {
             p.max_outer = std::atoi(v);
         } else if (opt(a, "--deff", &v)) {

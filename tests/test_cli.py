@@ -27,10 +27,17 @@ def test_too_few_positional_arguments():
     (("x", "29", "55.66", "1", "0"), "not a number"),
     (("30", "29", "55.66", "1", "0", "--nx=100"), "nx=100"),
     (("30", "29", "55.66", "1", "0", "--lz=0"), "lengths"),
+    (("30", "29", "55.66", "1", "0", "--waist-y=-3"), "waist_y_um"),
 ])
 def test_invalid_inputs_fail_before_any_gpu_work(args, msg):
     r = run(*args)
     assert r.returncode == 1 and msg in r.stderr
+
+
+@pytest.mark.parametrize("opt", ["--heat-bc=2", "--heat=gauss", "--scheme=euler", "--precision=fp16"])
+def test_unknown_choice_is_rejected(opt):
+    r = run("30", "29", "55.66", "1", "0", opt)
+    assert r.returncode == 2 and "is not one of" in r.stderr
 
 
 def test_unknown_option_and_crystal():
