@@ -1,7 +1,8 @@
 """GPU parity of the THERMAL path against the FP64 oracle.
 
-Tolerances: eta and peak dT relative error <= 1e-5 (north_star); the
-temperature field to 1e-5 of the peak rise (both solve to the same discrete
+Tolerances: eta and peak dT relative error <= 1e-5 (the accuracy bar of the
+FP32 batched form); the temperature field to 1e-5 of the peak rise (both
+solve to the same discrete
 steady state -- the FP32-mixed form refines its FP32 fast-diagonalisation
 solve with FP64 residuals until the update is < 1e-8 K -- but the heat
 source deposited by the FP32 fields differs from the FP64 one at ~1e-7).
