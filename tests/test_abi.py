@@ -144,3 +144,26 @@ def test_null_arguments_are_einval_not_crashes():
     assert L.shg_get_plan(None, 64, abi.PREC_FP32_MIXED, None, None) == abi.SHG_EINVAL
     assert L.shg_autotune(None, 64, abi.PREC_FP32_MIXED, 1, 8) == abi.SHG_EINVAL
     assert L.shg_set_plan(None, 64, abi.PREC_FP32_MIXED, 1, 4) == abi.SHG_EINVAL
+
+
+def test_plain_c_program_links_and_calls_the_library(tmp_path):
+    """A C99 consumer needs only include/shg.h and -lshg_hip (no HIP headers,
+    no C++): it links, and the host-only entry points answer as the Python
+    mirror does."""
+    src = tmp_path / "consumer.c"
+    src.write_text(
+        "#include <stdio.h>\n#include \"shg.h\"\n"
+        "int main(void) {\n"
+        "  shg_params p; shg_params_default(&p);\n"
+        "  printf(\"%d %d %d %d\\n\", p.nx, p.nz, shg_validate(&p), shg_validate(0));\n"
+        "  printf(\"%.17g\\n\", shg_refractive_index(SHG_CRYSTAL_MGO_SPPLT, 1.064, 25.0));\n"
+        "  printf(\"%d %d\\n\", shg_fft_size_supported(384), shg_fft_size_supported(100));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "consumer"
+    libdir = os.path.dirname(shg.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-L", libdir, "-lshg_hip",
+                    f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out[:4] == ["128", "100", "0", str(abi.SHG_EINVAL)]
+    assert float(out[4]) == pytest.approx(O.refractive_index(0, 1.064, 25.0), rel=1e-15)
+    assert out[5:] == ["1", "0"]
