@@ -272,9 +272,8 @@ static int march_lanes(shg_handle* h, RowGroups& rowg, ColGroups& colg, bool sin
 // group (an instance past its last step returns at once), so the captured
 // shapes do not change along the march and the graph is cached on the
 // handle, keyed by those shapes and pointers, for later marches.
-static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G) {
+static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G, int C) {
     if (!h->step_counter) SHG_HIP(hipMalloc(&h->step_counter, 256));
-    const int C = h->graph_chunk;
     Lanes L;
     if (int rc = march_lanes(h, rowg, colg, false, L)) return rc;
     struct Launch {
@@ -396,14 +395,18 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
     int G = 0;
     for (auto* m : act) G = std::max(G, m->nz);
     const bool traced = std::any_of(act.begin(), act.end(), [](const MixedHost* m) { return m->trace != nullptr; });
-    if (h->graph_chunk > 0 && !h->profiling && !traced) {
-        if (int rc = march_graph(h, rowg, colg, G)) return rc;
+    Lanes L;
+    if (int rc = march_lanes(h, rowg, colg, traced || h->profiling, L)) return rc;
+    // graph replay: as set, or automatically (32 steps per graph) when the
+    // march has independent lanes, where the replayed fork/join measured
+    // faster than direct launches
+    const int chunk = h->graph_chunk >= 0 ? h->graph_chunk : (L.n > 1 ? 32 : 0);
+    if (chunk > 0 && !h->profiling && !traced) {
+        if (int rc = march_graph(h, rowg, colg, G, chunk)) return rc;
         SHG_HIP(hipStreamSynchronize(h->stream));
         for (auto* m : act) m->steps += m->nz;
         return SHG_OK;
     }
-    Lanes L;
-    if (int rc = march_lanes(h, rowg, colg, traced || h->profiling, L)) return rc;
     hipStream_t* streams = L.streams;
     auto& lane = L.lane;
     const int nlanes = L.n;

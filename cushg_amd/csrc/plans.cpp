@@ -150,8 +150,8 @@ int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int
 
 int shg_set_graph_steps(shg_handle* h, int32_t steps) {
     if (!h) return SHG_EINVAL;
-    if (steps < 0 || steps > 4096) {
-        set_error("shg_set_graph_steps: steps=%d (0..4096)", steps);
+    if (steps < -1 || steps > 4096) {
+        set_error("shg_set_graph_steps: steps=%d (-1..4096)", steps);
         return SHG_EINVAL;
     }
     h->graph_chunk = steps;

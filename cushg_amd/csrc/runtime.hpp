@@ -95,9 +95,11 @@ struct shg_handle {
     // default
     std::map<std::pair<int, int>, int> rows_plan, cols_plan;
     // graph replay of the fused march: steps per captured graph (0: direct
-    // launches), the device step counter and the cached executable graphs
-    // keyed by the launch structure they were captured from
-    int graph_chunk = 0;
+    // launches; negative: automatic -- graphs of 32 steps for marches with
+    // more than one independent launch lane), the device step counter and
+    // the cached executable graphs keyed by the launch structure they were
+    // captured from
+    int graph_chunk = -1;
     int* step_counter = nullptr;
     std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
     // profiling

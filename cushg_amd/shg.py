@@ -228,7 +228,8 @@ class Handle:
         return self.get_plan(n, precision)
 
     def set_graph_steps(self, steps: int):
-        """Replay the fused march as hipGraphs of `steps` z-steps (0: direct launches)."""
+        """Replay the fused march as hipGraphs of `steps` z-steps (0: direct
+        launches; -1, the default: graphs for batches with independent lanes)."""
         _check(lib().shg_set_graph_steps(self._h, steps))
 
 

@@ -289,10 +289,12 @@ SHG_API int shg_get_plan(shg_handle* h, int32_t n, int32_t precision, int32_t* r
 SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int32_t nz);
 
 /* Graph replay of the fused march: `steps` > 0 captures the launches of
- * that many consecutive z-steps (every launch group's row and column pass)
- * once into a hipGraph, cached on the handle, and replays it along the
- * march; 0 launches every kernel directly.  Runs with a per-slice trace or
- * with profiling on always launch directly.  Replaces the reference's
+ * that many consecutive z-steps (every launch group's row and column pass;
+ * independent groups as parallel branches) once into a hipGraph, cached on
+ * the handle, and replays it along the march; 0 launches every kernel
+ * directly; -1 (the default) replays graphs of 32 steps for batches whose
+ * launch groups form independent lanes and launches directly otherwise.
+ * Runs with a per-slice trace or with profiling on always launch directly.  Replaces the reference's
  * per-slice host loop with its synchronisations (src/headers/Solver.h:276-308). */
 SHG_API int shg_set_graph_steps(shg_handle* h, int32_t steps);
 
