@@ -50,6 +50,7 @@ def lib():
         L.orc_heat_solve_direct.argtypes = [C.POINTER(ShgParams), C.c_int32, dp, dp]
         L.orc_tridiag_eigen.argtypes = [C.c_int32, dp, dp, dp, dp]
         L.orc_run.argtypes = [C.POINTER(ShgParams), C.POINTER(ShgResult)]
+        L.orc_march_heat.argtypes = [C.POINTER(ShgParams), dp, dp, dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -140,6 +141,17 @@ def heat_solve_direct(p: ShgParams, nzh: int, Q: np.ndarray) -> np.ndarray:
     rc = lib().orc_heat_solve_direct(C.byref(p), nzh, _dp(Q), _dp(T))
     assert rc == 0
     return T
+
+
+def march_heat(p: ShgParams, T: np.ndarray):
+    """orc_march_heat: one THERMAL march on T[nzh, ny, nx]; returns the heat
+    source Q[nzh, ny, nx] and the (pump, SH) powers after every slice."""
+    T = np.ascontiguousarray(T, np.float64)
+    Q = np.empty_like(T)
+    pw = np.empty((p.nz, 2))
+    rc = lib().orc_march_heat(C.byref(p), _dp(T), _dp(Q), _dp(pw))
+    assert rc == 0
+    return Q, pw
 
 
 def tridiag_eigen(diag: np.ndarray, off: np.ndarray):

@@ -854,6 +854,59 @@ static double now_s(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* One THERMAL z-march on a given temperature field T[nzh][ny][nx]: the
+ * heat source it deposits, Q[nzh][ny][nx] averaged over the optical slices
+ * of each heat slice as in the outer loop below (src/headers/Solver.h:
+ * 299-301, src/headers/Tfield.h:260-276), and the pump and SH powers after
+ * every slice, power_after[2 s] and power_after[2 s + 1] (n at the centre
+ * temperature, as the run's results).  Test infrastructure: pins Q against
+ * the power the fields lose. */
+int orc_march_heat(const shg_params* p, const double* T, double* Q, double* power_after) {
+    if (p->nx < 4 || p->ny < 4 || p->nz < 2 || (p->nx & 1) || (p->ny & 1)) return SHG_EINVAL;
+    grid_t g;
+    grid_init(p, &g);
+    if (g.nzh < 2) return SHG_EINVAL;
+    const size_t N = (size_t)g.nx * g.ny, NV = N * g.nzh;
+    fft_plan px, py;
+    if (fft_plan_init(&px, g.nx) || fft_plan_init(&py, g.ny)) return SHG_EINVAL;
+    cplx* P = (cplx*)calloc(N, sizeof(cplx));
+    cplx* S = (cplx*)calloc(N, sizeof(cplx));
+    cplx* prop = (cplx*)calloc(2 * N, sizeof(cplx));
+    double* qsum = (double*)calloc(NV, sizeof(double));
+    int* qcnt = (int*)calloc(g.nzh, sizeof(int));
+    double* dk = (double*)malloc(sizeof(double) * NV);
+    double* dkint = (double*)calloc(N, sizeof(double));
+    rk_call calls[2];
+    scheme_calls(p, g.dz, calls);
+    for (size_t i = 0; i < NV; i++) dk[i] = orc_delta_k(p, T[i]);
+    const double Tc = T[I3(g.nx / 2, g.ny / 2, 0)];
+    orc_pump_init(p, Tc, (double*)P);
+    orc_propagators(p, Tc, (double*)prop, (double*)(prop + N));
+    for (int s = 0; s < g.nz; s++) {
+        const int h = (int)(((long)s * g.nzh) / g.nz);
+        const double* Th = T + (size_t)h * N;
+        const double* dkh = dk + (size_t)h * N;
+        for (size_t i = 0; i < N; i++) dkint[i] += dkh[i];
+        rk4_thermal(p, &g, s, Th, dkint, &calls[0], P, S);
+        diffraction(&g, &px, &py, P, prop);
+        diffraction(&g, &px, &py, S, prop + N);
+        rk4_thermal(p, &g, s, Th, dkint, &calls[1], P, S);
+        double* qh = qsum + (size_t)h * N;
+        for (size_t i = 0; i < N; i++) qh[i] += heat_source(p, &g, Th[i], P[i], S[i]);
+        qcnt[h]++;
+        if (power_after) {
+            power_after[2 * s] = orc_power(p, (double*)P, orc_refractive_index(p->crystal, g.lp, Tc));
+            power_after[2 * s + 1] = orc_power(p, (double*)S, orc_refractive_index(p->crystal, g.ls, Tc));
+        }
+    }
+    for (int h = 0; h < g.nzh; h++)
+        for (size_t i = 0; i < N; i++) Q[(size_t)h * N + i] = qcnt[h] ? qsum[(size_t)h * N + i] / qcnt[h] : 0.0;
+    free(P); free(S); free(prop); free(qsum); free(qcnt); free(dk); free(dkint);
+    fft_plan_free(&px);
+    fft_plan_free(&py);
+    return SHG_OK;
+}
+
 int orc_run(const shg_params* p, shg_result* r) {
     const double t0 = now_s();
     if (p->nx < 4 || p->ny < 4 || p->nz < 2 || (p->nx & 1) || (p->ny & 1)) return SHG_EINVAL;

@@ -50,6 +50,9 @@ int orc_tridiag_eigen(int32_t n, const double* diag, const double* off,
                       double* evals, double* evecs);
 
 /* full run = Solver::run */
+/* one THERMAL march on a given T: heat source per heat slice and the pump
+ * and SH powers after every slice */
+int orc_march_heat(const shg_params* p, const double* T, double* Q, double* power_after);
 int orc_run(const shg_params* p, shg_result* r);
 
 #ifdef __cplusplus

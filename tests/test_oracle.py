@@ -13,7 +13,9 @@ wrong sign or index, transposed operand) fails it:
     assembled finite-volume system solved by scipy;
   - the THERMAL march with a uniform temperature against the non-THERMAL
     march (pins the running dk integral and the heat-slice mapping);
-  - the reference Jacobi loop against the direct solver.
+  - the reference Jacobi loop against the direct solver;
+  - the heat source against the exponential pump decay and the power the
+    fields lose slice by slice (energy balance with two-photon absorption).
 """
 import json
 import math
@@ -390,3 +392,58 @@ def test_thermal_jacobi_reference_loop_vs_direct():
     assert np.abs(oj["T"] - od["T"]).max() < 1e-8
     assert rj.eta == pytest.approx(rd.eta, rel=1e-9)
     assert rj.jacobi_sweeps > rd.jacobi_sweeps
+
+
+def _march_heat_setup(nz, nzh):
+    p = configs.spplt64()
+    p.scheme = abi.SCHEME_STRANG
+    p.thermal = 1
+    p.power_W = 50.0
+    p.nz, p.nz_heat = nz, nzh
+    p.alpha_p, p.alpha_s, p.beta_s = 2e-5, 2e-4, 1.0
+    dA = (p.lx_um / (p.nx - 1)) * (p.ly_um / (p.ny - 1))
+    dz = p.lz_um / (p.nz - 1)
+    n_p = O.refractive_index(p.crystal, p.lambda_p_um, p.T_C)
+    P_in = O.power(p, O.pump_init(p, p.T_C), n_p)
+    return p, dA, dz, P_in, np.full((nzh, p.ny, p.nx), p.T_C)
+
+
+def test_heat_source_without_conversion_is_the_absorbed_pump():
+    """deff = 0: the pump decays as exp(-alpha_p z), so the heat of optical
+    slice s integrated over the plane is alpha_p P_in exp(-alpha_p (s+1) dz);
+    a heat slice holds the mean over the optical slices s with
+    floor(s nzh / nz) = h.  Pins Q's alpha_p I_p term (the 1/2 c eps0 n_p
+    intensity) against the beam power and the slice-to-heat-slab averaging."""
+    p, dA, dz, P_in, T = _march_heat_setup(40, 7)
+    p.deff_um_per_V = 0.0
+    Q, pw = O.march_heat(p, T)
+    assert np.all(pw[:, 1] == 0.0)
+    s = np.arange(p.nz)
+    per_slice = p.alpha_p * P_in * np.exp(-p.alpha_p * (s + 1) * dz)
+    slab = (s * p.nz_heat) // p.nz
+    want = np.array([per_slice[slab == h].mean() for h in range(p.nz_heat)])
+    got = Q.sum(axis=(1, 2)) * dA
+    assert np.abs(got / want - 1).max() < 1e-9
+    # uniform in z within a slab only through the average: the first and last
+    # slabs differ by the decay over the crystal
+    assert got[0] / got[-1] == pytest.approx(want[0] / want[-1], rel=1e-12)
+
+
+def test_heat_source_closes_the_energy_balance():
+    """With conversion, linear SH absorption and two-photon absorption: the
+    power the two fields lose over a slice equals the heat deposited in it
+    (trapezoid of the plane-integrated Q at the slice ends; the z = 0 end
+    holds only the pump).  alpha_p I_p, alpha_s I_s and beta_s I_s^2 each
+    carry more than 5% of the heat, so a wrong intensity factor (n_s for
+    n_p, a dropped 1/2, I_s for I_s^2) fails it."""
+    p, dA, dz, P_in, T = _march_heat_setup(120, 120)
+    Q, pw = O.march_heat(p, T)
+    H = Q.sum(axis=(1, 2)) * dA
+    ends = np.concatenate([[p.alpha_p * P_in], H])
+    total = np.concatenate([[P_in], pw.sum(axis=1)])
+    loss = total[:-1] - total[1:]
+    assert np.abs(loss / (0.5 * dz * (ends[:-1] + ends[1:])) - 1).max() < 4e-4
+    share_p = p.alpha_p * pw[:, 0].sum() / H.sum()
+    share_s = p.alpha_s * pw[:, 1].sum() / H.sum()
+    assert share_p > 0.05 and share_s > 0.05 and 1 - share_p - share_s > 0.05
+    assert pw[-1, 1] > 1e-3 * p.power_W
