@@ -102,11 +102,14 @@ template <> __device__ __forceinline__ double recip<double>(float n) {
 // dA/dz of src/headers/Solver.h:57-80 (and :33-55 for THERMAL):
 //   dP = i kp S conj(P) conj(e) - alpha_p/2 P
 //   dS = i ks P^2 e - (alpha_s/2 + bfac/2 |S|^2) S
+// Both coupling terms go through u = P e: S conj(P) conj(e) = S conj(u) and
+// P^2 e = P u (three complex products instead of four).
 template <typename C, typename R = re_t<C>>
 __device__ __forceinline__ void dadz(C P, C S, C e, R kp, R ks, R aph, R ash, R bfac, C& dP, C& dS) {
-    const C spc = cmul(cmul(S, conjc(P)), conjc(e));
+    const C u = cmul(P, e);
+    const C spc = cmul(S, conjc(u));
     dP = mk<C>(-kp * spc.y - aph * P.x, kp * spc.x - aph * P.y);
-    const C ppe = cmul(cmul(P, P), e);
+    const C ppe = cmul(P, u);
     const R g = ash + R(0.5) * bfac * abs2(S);
     dS = mk<C>(-ks * ppe.y - g * S.x, ks * ppe.x - g * S.y);
 }
