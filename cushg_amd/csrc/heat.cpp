@@ -135,15 +135,25 @@ static bool sym_tridiag_eig(int n, std::vector<double>& d, std::vector<double> e
 // One axis of the finite-volume form (values divided by k_th): node widths
 // w, stiffness diagonal k (1/h per neighbour link, + h_conv/k_th at Robin
 // ends), eigenpairs of the symmetric W^-1/2 K W^-1/2.
+//
+// Folded axis (both ends of the same kind, even length n = 2m): the matrix
+// is then persymmetric, its eigenvectors are [a; Ja]/sqrt2 (even) or
+// [a; -Ja]/sqrt2 (odd, J the reversal), and a runs over the eigenvectors of
+// the m x m leading block with the coupling o_{m-1} across the middle added
+// to (even) or subtracted from (odd) its last diagonal entry.  q then holds
+// the even basis (m x m) followed by the odd one, lam the even eigenvalues
+// followed by the odd ones.
 struct AxisHost {
     int n = 0;
+    bool fold = false;
     std::vector<double> k, w, lam, q;
 };
 // An end marked Dirichlet couples its node to a fixed value one spacing
 // beyond it (a removed unknown, full-width control volume).
 static bool axis_build(int n, double h, bool robin_lo, bool robin_hi, bool dirichlet_lo, bool dirichlet_hi, double hk,
-                       AxisHost& a) {
+                       bool foldable, AxisHost& a) {
     a.n = n;
+    a.fold = foldable && n >= 4 && n % 2 == 0 && robin_lo == robin_hi && dirichlet_lo == dirichlet_hi;
     a.k.resize(n);
     a.w.resize(n);
     std::vector<double> diag(n), off(std::max(n - 1, 1), 0.0);
@@ -159,14 +169,28 @@ static bool axis_build(int n, double h, bool robin_lo, bool robin_hi, bool diric
         diag[i] = kd / a.w[i];
     }
     for (int i = 0; i + 1 < n; i++) off[i] = -1.0 / h / std::sqrt(a.w[i] * a.w[i + 1]);
-    a.lam = diag;
-    return sym_tridiag_eig(n, a.lam, off, a.q);
+    if (!a.fold) {
+        a.lam = diag;
+        return sym_tridiag_eig(n, a.lam, off, a.q);
+    }
+    const int m = n / 2;
+    a.lam.clear();
+    a.q.clear();
+    for (int sign : {1, -1}) {
+        std::vector<double> d(diag.begin(), diag.begin() + m), e(off.begin(), off.begin() + (m - 1)), z;
+        d[m - 1] += sign * off[m - 1];
+        if (!sym_tridiag_eig(m, d, e, z)) return false;
+        a.lam.insert(a.lam.end(), d.begin(), d.end());
+        a.q.insert(a.q.end(), z.begin(), z.end());
+    }
+    return true;
 }
 
 // Device-side tables of the direct solver (carved from the run arena).
 struct DirectTables {
     HeatAxes ax{};
     const void* q[3] = {nullptr, nullptr, nullptr};  // eigenvector matrices (R precision)
+    bool fold[3] = {false, false, false};           // axis transformed in its even/odd fold
     void* u = nullptr;
     void* v = nullptr;
     bool fp64 = false;
@@ -177,6 +201,7 @@ struct DirectTables {
 // on the handle: the host eigen-decompositions are O(n^3).
 struct HeatGeom {
     HeatAxes ax{};
+    bool fold[3] = {};
     void* mem[9] = {};
     double* q64[3] = {};
     float* q32[3] = {};
@@ -194,15 +219,16 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
     auto it = h->heat_cache.find(key);
     if (it != h->heat_cache.end()) return it->second;
     // open top: x (Robin both ends), y' = y-1 (Dirichlet below, Robin top), z (Robin both);
-    // oven-surrounded: x' = x-1 and y' = y-1 (Dirichlet both ends), z (Robin both)
+    // oven-surrounded: x' = x-1 and y' = y-1 (Dirichlet both ends), z (Robin both).
+    // x and z are folded when their length is even (y is transformed whole).
     const double hk = p->h_conv / p->k_th;
     const bool oven = p->heat_bc == SHG_HEAT_BC_OVEN_SURROUNDED;
     AxisHost axes[3];
-    const bool ok = (oven ? axis_build(hs.nx - 2, hs.dx, false, false, true, true, hk, axes[0])
-                          : axis_build(hs.nx, hs.dx, true, true, false, false, hk, axes[0])) &&
-                    (oven ? axis_build(hs.ny - 2, hs.dy, false, false, true, true, hk, axes[1])
-                          : axis_build(hs.ny - 1, hs.dy, false, true, true, false, hk, axes[1])) &&
-                    axis_build(hs.nzh, hs.dz, true, true, false, false, hk, axes[2]);
+    const bool ok = (oven ? axis_build(hs.nx - 2, hs.dx, false, false, true, true, hk, true, axes[0])
+                          : axis_build(hs.nx, hs.dx, true, true, false, false, hk, true, axes[0])) &&
+                    (oven ? axis_build(hs.ny - 2, hs.dy, false, false, true, true, hk, false, axes[1])
+                          : axis_build(hs.ny - 1, hs.dy, false, true, true, false, hk, false, axes[1])) &&
+                    axis_build(hs.nzh, hs.dz, true, true, false, false, hk, true, axes[2]);
     if (!ok) {
         set_error("heat: tridiagonal eigen-solver did not converge");
         return nullptr;
@@ -214,6 +240,8 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
     bool fail = false;
     for (int i = 0; i < 3 && !fail; i++) {
         const int n = axes[i].n;
+        const size_t nq = axes[i].q.size();  // n*n, or 2 (n/2)^2 folded
+        g->fold[i] = axes[i].fold;
         double* d = nullptr;
         fail |= hipMalloc(reinterpret_cast<void**>(&d), 8 * 3 * (size_t)n) != hipSuccess;
         if (fail) break;
@@ -226,15 +254,15 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
         *lp[i] = d + 2 * n;
         double* q64 = nullptr;
         float* q32 = nullptr;
-        fail |= hipMalloc(reinterpret_cast<void**>(&q64), 8 * (size_t)n * n) != hipSuccess;
+        fail |= hipMalloc(reinterpret_cast<void**>(&q64), 8 * nq) != hipSuccess;
         if (fail) break;
         g->mem[3 * i + 1] = q64;
-        fail |= hipMalloc(reinterpret_cast<void**>(&q32), 4 * (size_t)n * n) != hipSuccess;
+        fail |= hipMalloc(reinterpret_cast<void**>(&q32), 4 * nq) != hipSuccess;
         if (fail) break;
         g->mem[3 * i + 2] = q32;
         std::vector<float> qf(axes[i].q.begin(), axes[i].q.end());
-        fail |= hipMemcpy(q64, axes[i].q.data(), 8 * (size_t)n * n, hipMemcpyHostToDevice) != hipSuccess;
-        fail |= hipMemcpy(q32, qf.data(), 4 * (size_t)n * n, hipMemcpyHostToDevice) != hipSuccess;
+        fail |= hipMemcpy(q64, axes[i].q.data(), 8 * nq, hipMemcpyHostToDevice) != hipSuccess;
+        fail |= hipMemcpy(q32, qf.data(), 4 * nq, hipMemcpyHostToDevice) != hipSuccess;
         g->q64[i] = q64;
         g->q32[i] = q32;
     }
@@ -325,6 +353,7 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     if (!g) return SHG_ENOCONV;
     dt->ax = g->ax;
     for (int i = 0; i < 3; i++) {
+        dt->fold[i] = g->fold[i];
         if (f64) {
             if (!g->q64[i]) return SHG_ENOMEM;
             dt->q[i] = g->q64[i];
@@ -355,33 +384,50 @@ static int read_scalar(shg_handle* h, const double* d, double* out) {
 }
 
 // out[..i..] = sum_j M[j][i] in[..j..] along one axis of u[nz][nyu][nx]
-// (M = Q forward, Q^T inverse; Q row-major, Q[j*n+i] = component j of vector i)
+// (M = Q forward, Q^T inverse; Q row-major, Q[j*n+i] = component j of vector
+// i).  On a folded axis (x or z) the data are in the even/odd fold and Q is
+// the even block followed by the odd one: two products of half the length,
+// one batched call.
 template <typename R>
-static int axis_gemm(shg_handle* h, int axis, bool fwd, const R* q, const R* in, R* out, int nx, int nyu, int nz) {
+static rocblas_status gemm_sb(rocblas_handle bh, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                              const R* a, int lda, rocblas_stride sa, const R* b, int ldb, rocblas_stride sb, R* c,
+                              int ldc, rocblas_stride sc, int batch) {
+    const R one = 1, zero = 0;
+    if (batch == 1) {
+        if constexpr (sizeof(R) == 4)
+            return rocblas_sgemm(bh, ta, tb, m, n, k, &one, a, lda, b, ldb, &zero, c, ldc);
+        else
+            return rocblas_dgemm(bh, ta, tb, m, n, k, &one, a, lda, b, ldb, &zero, c, ldc);
+    }
+    if constexpr (sizeof(R) == 4)
+        return rocblas_sgemm_strided_batched(bh, ta, tb, m, n, k, &one, a, lda, sa, b, ldb, sb, &zero, c, ldc, sc,
+                                             batch);
+    else
+        return rocblas_dgemm_strided_batched(bh, ta, tb, m, n, k, &one, a, lda, sa, b, ldb, sb, &zero, c, ldc, sc,
+                                             batch);
+}
+
+template <typename R>
+static int axis_gemm(shg_handle* h, int axis, bool fwd, bool fold, const R* q, const R* in, R* out, int nx, int nyu,
+                     int nz) {
     rocblas_handle bh = blas(h);
     if (!bh) {
         set_error("rocBLAS handle creation failed");
         return SHG_EDEVICE;
     }
-    const R one = 1, zero = 0;
     rocblas_status st;
     const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
-    if constexpr (sizeof(R) == 4) {
-        if (axis == 0)
-            st = rocblas_sgemm(bh, fwd ? N_ : T_, N_, nx, nyu * nz, nx, &one, q, nx, in, nx, &zero, out, nx);
-        else if (axis == 1)
-            st = rocblas_sgemm_strided_batched(bh, N_, fwd ? T_ : N_, nx, nyu, nyu, &one, in, nx, (rocblas_stride)nx * nyu,
-                                               q, nyu, 0, &zero, out, nx, (rocblas_stride)nx * nyu, nz);
-        else
-            st = rocblas_sgemm(bh, N_, fwd ? T_ : N_, nx * nyu, nz, nz, &one, in, nx * nyu, q, nz, &zero, out, nx * nyu);
+    const rocblas_stride plane = (rocblas_stride)nx * nyu;
+    if (axis == 0) {
+        const int n = fold ? nx / 2 : nx;  // per block
+        st = gemm_sb<R>(bh, fwd ? N_ : T_, N_, n, nyu * nz, n, q, n, (rocblas_stride)n * n, in, nx, n, out, nx, n,
+                        fold ? 2 : 1);
+    } else if (axis == 1) {
+        st = gemm_sb<R>(bh, N_, fwd ? T_ : N_, nx, nyu, nyu, in, nx, plane, q, nyu, 0, out, nx, plane, nz);
     } else {
-        if (axis == 0)
-            st = rocblas_dgemm(bh, fwd ? N_ : T_, N_, nx, nyu * nz, nx, &one, q, nx, in, nx, &zero, out, nx);
-        else if (axis == 1)
-            st = rocblas_dgemm_strided_batched(bh, N_, fwd ? T_ : N_, nx, nyu, nyu, &one, in, nx, (rocblas_stride)nx * nyu,
-                                               q, nyu, 0, &zero, out, nx, (rocblas_stride)nx * nyu, nz);
-        else
-            st = rocblas_dgemm(bh, N_, fwd ? T_ : N_, nx * nyu, nz, nz, &one, in, nx * nyu, q, nz, &zero, out, nx * nyu);
+        const int n = fold ? nz / 2 : nz;
+        st = gemm_sb<R>(bh, N_, fwd ? T_ : N_, nx * nyu, n, n, in, nx * nyu, plane * n, q, n, (rocblas_stride)n * n,
+                        out, nx * nyu, plane * n, fold ? 2 : 1);
     }
     if (st != rocblas_status_success) {
         set_error("rocBLAS gemm failed (%d) on axis %d", (int)st, axis);
@@ -406,24 +452,29 @@ static int resid_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTab
 template <typename R>
 static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* max_corr) {
     const int nx = c.nxu, nyu = c.nyu, nz = hs.nzh;
-    R* u = static_cast<R*>(dt.u);
-    R* v = static_cast<R*>(dt.v);
+    const bool fx = dt.fold[0], fz = dt.fold[2];
+    R* a = static_cast<R*>(dt.u);
+    R* b = static_cast<R*>(dt.v);
     const R* qx = static_cast<const R*>(dt.q[0]);
     const R* qy = static_cast<const R*>(dt.q[1]);
     const R* qz = static_cast<const R*>(dt.q[2]);
     {
         ProfScope ps(h, "heat_fd_gemm");
-        if (int rc = axis_gemm<R>(h, 0, true, qx, u, v, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 1, true, qy, v, u, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 2, true, qz, u, v, nx, nyu, nz)) return rc;
-        SHG_HIP(launch_fd_divide<R>(h->stream, v, dt.ax, nx, nyu, nz));
-        if (int rc = axis_gemm<R>(h, 2, false, qz, v, u, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 1, false, qy, u, v, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 0, false, qx, v, u, nx, nyu, nz)) return rc;
+        if (fx || fz) {
+            SHG_HIP(launch_fd_fold<R>(h->stream, a, b, nx, nyu, nz, fx, fz));
+            std::swap(a, b);
+        }
+        if (int rc = axis_gemm<R>(h, 0, true, fx, qx, a, b, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 1, true, false, qy, b, a, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 2, true, fz, qz, a, b, nx, nyu, nz)) return rc;
+        SHG_HIP(launch_fd_divide<R>(h->stream, b, dt.ax, nx, nyu, nz));
+        if (int rc = axis_gemm<R>(h, 2, false, fz, qz, b, a, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 1, false, false, qy, a, b, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm<R>(h, 0, false, fx, qx, b, a, nx, nyu, nz)) return rc;
     }
     {
         ProfScope ps(h, "heat_update");
-        SHG_HIP(launch_fd_update<R>(h->stream, c, dt.ax, u, hs.T, hs.part));
+        SHG_HIP(launch_fd_update<R>(h->stream, c, dt.ax, a, hs.T, hs.part, fx, fz));
         SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, true, hs.red + 1));
     }
     return read_scalar(h, hs.red + 1, max_corr);

@@ -24,7 +24,8 @@ struct HeatCoef {
 // 1D factors of the tensor-product finite-volume form
 //   A = Kx (x) Wy (x) Wz + Wx (x) Ky (x) Wz + Wx (x) Wy (x) Kz
 // (kx/ky/kz: diagonals of K; wx/wy/wz: control-volume widths; lx/ly/lz:
-// eigenvalues of W^-1/2 K W^-1/2), over the unknown block of HeatCoef.
+// eigenvalues of W^-1/2 K W^-1/2, even then odd ones on a folded axis),
+// over the unknown block of HeatCoef.
 struct HeatAxes {
     const double *kx, *ky, *kz;
     const double *wx, *wy, *wz;
@@ -41,9 +42,13 @@ hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& 
                              R* u, double* part_sq);
 template <typename R>
 hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz);
+// even/odd fold of the x and/or z axis of u (in -> out; not in place)
+template <typename R>
+hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz);
+// T += W^-1/2 u, u folded along x / z as launch_fd_fold leaves it
 template <typename R>
 hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
-                            double* part_max);
+                            double* part_max, bool fx, bool fz);
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out);
 hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel,
                             bool oven);

@@ -181,22 +181,89 @@ __global__ __launch_bounds__(256) void k_fd_divide(R* __restrict__ v, const doub
     v[i] = (R)((double)v[i] / (lx[x] + ly[j] + lz[z]));
 }
 
-// T += W^-1/2 u (the correction); partial max |correction|
+// Even/odd fold of the symmetric axes of u[nz][nyu][nxu]: along a folded
+// axis of even length n = 2m, entry i < m becomes (a_i + a_{n-1-i})/sqrt2 and
+// entry m + i becomes (a_i - a_{n-1-i})/sqrt2.  The fold is orthogonal; in it
+// the axis operator is block diagonal (even and odd eigenvectors), so each
+// transform along that axis is two m x m products instead of one n x n.
+// fx / fz: fold x / z.  grid (ceil(hx/64), ceil(nyu/4), hz), block (64, 4),
+// with hx, hz the folded half lengths (the full length on an axis not folded).
+constexpr double kRsqrt2 = 0.70710678118654752440;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fd_fold(const R* __restrict__ in, R* __restrict__ out, int nxu, int nyu,
+                                                 int nz, int fx, int fz) {
+    const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
+    const int xu = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (xu >= hx || j >= nyu || z >= hz) return;
+    const int xm = nxu - 1 - xu, zm = nz - 1 - z;
+    auto at = [&](int zi, int xi) -> size_t { return ((size_t)zi * nyu + j) * nxu + xi; };
+    if (fx && fz) {
+        const double a = in[at(z, xu)], b = in[at(z, xm)], c = in[at(zm, xu)], d = in[at(zm, xm)];
+        const double ez = (a + b) * kRsqrt2, oz = (a - b) * kRsqrt2, ezm = (c + d) * kRsqrt2, ozm = (c - d) * kRsqrt2;
+        out[at(z, xu)] = (R)((ez + ezm) * kRsqrt2);
+        out[at(hz + z, xu)] = (R)((ez - ezm) * kRsqrt2);
+        out[at(z, hx + xu)] = (R)((oz + ozm) * kRsqrt2);
+        out[at(hz + z, hx + xu)] = (R)((oz - ozm) * kRsqrt2);
+    } else if (fx) {
+        const double a = in[at(z, xu)], b = in[at(z, xm)];
+        out[at(z, xu)] = (R)((a + b) * kRsqrt2);
+        out[at(z, hx + xu)] = (R)((a - b) * kRsqrt2);
+    } else if (fz) {
+        const double a = in[at(z, xu)], c = in[at(zm, xu)];
+        out[at(z, xu)] = (R)((a + c) * kRsqrt2);
+        out[at(hz + z, xu)] = (R)((a - c) * kRsqrt2);
+    } else {
+        out[at(z, xu)] = in[at(z, xu)];
+    }
+}
+
+// T += W^-1/2 u (the correction), u unfolded first along the folded axes
+// (the inverse of k_fd_fold); partial max |correction|.  A thread owns the
+// up to four nodes (xu | nxu-1-xu) x (z | nz-1-z) that one folded entry set
+// maps to.  grid plane_grid: x blocks over the half length hx, y over the
+// node rows, z chunks over hz.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_update(HeatCoef c, HeatAxes a, const R* __restrict__ u,
-                                                   double* __restrict__ T, double* __restrict__ part_max) {
+                                                   double* __restrict__ T, double* __restrict__ part_max, int fx,
+                                                   int fz) {
     __shared__ double red[4];
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
-    const ZRange zr = zchunk(c);
+    const int nxu = c.nxu, nyu = c.nyu, nz = c.nz;
+    const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
+    const int xu = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, j = y - 1;
+    const int per = (hz + gridDim.z - 1) / gridDim.z, z0 = blockIdx.z * per, z1 = min(hz, z0 + per);
     double m = 0.0;
-    const int xu = x - c.x0, j = y - 1;
-    if (xu >= 0 && xu < c.nxu && j >= 0 && j < c.nyu) {
-        const double wxy = a.wx[xu] * a.wy[j];
-        const size_t nyu = (size_t)c.nyu, nxu = (size_t)c.nxu;
-        for (int z = zr.z0; z < zr.z1; z++) {
-            const double corr = (double)u[((size_t)z * nyu + j) * nxu + xu] * rsqrt(wxy * a.wz[z]);
-            T[((size_t)z * c.ny + y) * c.nx + x] += corr;
+    if (xu < hx && j >= 0 && j < nyu) {
+        const int xm = nxu - 1 - xu;
+        const double wy = a.wy[j];
+        auto at = [&](int zi, int xi) -> size_t { return ((size_t)zi * nyu + j) * nxu + xi; };
+        auto apply = [&](int zi, int xi, double val) {
+            const double corr = val * rsqrt(a.wx[xi] * wy * a.wz[zi]);
+            T[((size_t)zi * c.ny + y) * c.nx + xi + c.x0] += corr;
             m = fmax(m, fabs(corr));
+        };
+        for (int z = z0; z < z1; z++) {
+            const int zm = nz - 1 - z;
+            if (fx && fz) {
+                const double ee = u[at(z, xu)], eo = u[at(hz + z, xu)], oe = u[at(z, hx + xu)],
+                             oo = u[at(hz + z, hx + xu)];
+                const double az = (ee + eo) * kRsqrt2, azm = (ee - eo) * kRsqrt2, bz = (oe + oo) * kRsqrt2,
+                             bzm = (oe - oo) * kRsqrt2;
+                apply(z, xu, (az + bz) * kRsqrt2);
+                apply(z, xm, (az - bz) * kRsqrt2);
+                apply(zm, xu, (azm + bzm) * kRsqrt2);
+                apply(zm, xm, (azm - bzm) * kRsqrt2);
+            } else if (fx) {
+                const double e = u[at(z, xu)], o = u[at(z, hx + xu)];
+                apply(z, xu, (e + o) * kRsqrt2);
+                apply(z, xm, (e - o) * kRsqrt2);
+            } else if (fz) {
+                const double e = u[at(z, xu)], o = u[at(hz + z, xu)];
+                apply(z, xu, (e + o) * kRsqrt2);
+                apply(zm, xu, (e - o) * kRsqrt2);
+            } else {
+                apply(z, xu, (double)u[at(z, xu)]);
+            }
         }
     }
     const double s = block_reduce(m, true, red);
@@ -321,15 +388,26 @@ template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&
 template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxes&, int, int, int);
 
 template <typename R>
+hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz) {
+    const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
+    hipLaunchKernelGGL((k_fd_fold<R>), dim3((hx + 63) / 64, (nyu + 3) / 4, hz), dim3(64, 4), 0, st, in, out, nxu, nyu,
+                       nz, fx ? 1 : 0, fz ? 1 : 0);
+    return hipGetLastError();
+}
+template hipError_t launch_fd_fold<float>(hipStream_t, const float*, float*, int, int, int, bool, bool);
+template hipError_t launch_fd_fold<double>(hipStream_t, const double*, double*, int, int, int, bool, bool);
+
+template <typename R>
 hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
-                            double* part_max) {
-    hipLaunchKernelGGL((k_fd_update<R>), plane_grid(c), dim3(64, 4), 0, st, c, a, u, T, part_max);
+                            double* part_max, bool fx, bool fz) {
+    hipLaunchKernelGGL((k_fd_update<R>), plane_grid(c), dim3(64, 4), 0, st, c, a, u, T, part_max, fx ? 1 : 0,
+                       fz ? 1 : 0);
     return hipGetLastError();
 }
 template hipError_t launch_fd_update<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const float*, double*,
-                                            double*);
+                                            double*, bool, bool);
 template hipError_t launch_fd_update<double>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*, double*,
-                                             double*);
+                                             double*, bool, bool);
 
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out) {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, part, n, is_max ? 1 : 0, out);
