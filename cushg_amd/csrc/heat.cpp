@@ -474,8 +474,9 @@ static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectT
     }
     {
         ProfScope ps(h, "heat_update");
-        SHG_HIP(launch_fd_update<R>(h->stream, c, dt.ax, a, hs.T, hs.part, fx, fz));
-        SHG_HIP(launch_reduce(h->stream, hs.part, hs.nblocks, true, hs.red + 1));
+        int nb = 0;
+        SHG_HIP(launch_fd_update<R>(h->stream, c, dt.ax, a, hs.T, hs.part, fx, fz, &nb));
+        SHG_HIP(launch_reduce(h->stream, hs.part, nb, true, hs.red + 1));
     }
     return read_scalar(h, hs.red + 1, max_corr);
 }

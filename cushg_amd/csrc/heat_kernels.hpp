@@ -45,10 +45,11 @@ hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, in
 // even/odd fold of the x and/or z axis of u (in -> out; not in place)
 template <typename R>
 hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz);
-// T += W^-1/2 u, u folded along x / z as launch_fd_fold leaves it
+// T += W^-1/2 u, u folded along x / z as launch_fd_fold leaves it; writes
+// *nblocks partial maxima (at most heat_grid_blocks(c))
 template <typename R>
 hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
-                            double* part_max, bool fx, bool fz);
+                            double* part_max, bool fx, bool fz, int* nblocks);
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out);
 hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel,
                             bool oven);

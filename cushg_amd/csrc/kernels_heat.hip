@@ -221,8 +221,8 @@ __global__ __launch_bounds__(256) void k_fd_fold(const R* __restrict__ in, R* __
 // T += W^-1/2 u (the correction), u unfolded first along the folded axes
 // (the inverse of k_fd_fold); partial max |correction|.  A thread owns the
 // up to four nodes (xu | nxu-1-xu) x (z | nz-1-z) that one folded entry set
-// maps to.  grid plane_grid: x blocks over the half length hx, y over the
-// node rows, z chunks over hz.
+// maps to.  grid: x blocks over the half length hx, y over the node rows, z
+// chunks over hz.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_update(HeatCoef c, HeatAxes a, const R* __restrict__ u,
                                                    double* __restrict__ T, double* __restrict__ part_max, int fx,
@@ -397,17 +397,24 @@ hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu,
 template hipError_t launch_fd_fold<float>(hipStream_t, const float*, float*, int, int, int, bool, bool);
 template hipError_t launch_fd_fold<double>(hipStream_t, const double*, double*, int, int, int, bool, bool);
 
+// The grid keeps the block count of plane_grid (the size of the partials
+// array) at most: the z chunks take the x blocks a fold saves.
 template <typename R>
 hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const R* u, double* T,
-                            double* part_max, bool fx, bool fz) {
-    hipLaunchKernelGGL((k_fd_update<R>), plane_grid(c), dim3(64, 4), 0, st, c, a, u, T, part_max, fx ? 1 : 0,
-                       fz ? 1 : 0);
+                            double* part_max, bool fx, bool fz, int* nblocks) {
+    const dim3 pg = plane_grid(c);
+    const int hx = fx ? c.nxu / 2 : c.nxu, hz = fz ? c.nz / 2 : c.nz;
+    const int bx = (hx + 63) / 64;
+    const int zc = std::max(1, std::min(hz, (int)(pg.x * pg.z) / bx));
+    const dim3 grid(bx, pg.y, zc);
+    *nblocks = (int)(grid.x * grid.y * grid.z);
+    hipLaunchKernelGGL((k_fd_update<R>), grid, dim3(64, 4), 0, st, c, a, u, T, part_max, fx ? 1 : 0, fz ? 1 : 0);
     return hipGetLastError();
 }
 template hipError_t launch_fd_update<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const float*, double*,
-                                            double*, bool, bool);
+                                            double*, bool, bool, int*);
 template hipError_t launch_fd_update<double>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*, double*,
-                                             double*, bool, bool);
+                                             double*, bool, bool, int*);
 
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out) {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, part, n, is_max ? 1 : 0, out);
