@@ -6,9 +6,10 @@ nx*ny*(z-steps executed) / wall time of the fused, batched FP32-field form,
 everything resident on the device (the pump is initialised on the GPU;
 only scalar results cross PCIe in the timed region).  One benchmark "step"
 is one complete run of the workload: every thermo-optic pass, heat solve
-and z-march of every instance.  The timed form is SHG_PREC_FP32_STORAGE
-(FP32 fields in HBM, FP64 arithmetic in registers) by default; the FP32-
-arithmetic form (SHG_PREC_FP32_MIXED) is timed beside it.
+and z-march of every instance.  The timed form is SHG_PREC_FP32_SPLIT by
+default (FP32 fields in HBM; the row pass in FP32 arithmetic, the column
+pass in FP64 arithmetic); SHG_PREC_FP32_STORAGE (FP64 arithmetic in both
+passes) is timed beside it.
 
 Default workload: 16 THERMAL MgO:PPLN instances -- 512x512 grid, 4000
 z-steps, 50 mm crystal, 30 W, heat grid 512x512x400 -- over seeded
@@ -149,12 +150,16 @@ def kernel_roofline(h, ps):
     base, size = parts[0], int(parts[1])
     thermal = "_thermal" in name
     f64 = name.endswith("_f64")
+    # arithmetic of each pass per form: FP64 in both (storage), FP64 in the
+    # column pass only (split), FP32 in both (mixed)
+    f64_forms = {"rowpass": (abi.PREC_FP32_STORAGE,),
+                 "colpass": (abi.PREC_FP32_STORAGE, abi.PREC_FP32_SPLIT)}.get(base, ())
     if base not in BYTES_PER_PT:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}, kt
     total = 0.0
     for p, r in zip(ps, res):
-        if (p.precision == abi.PREC_FP32_STORAGE) != f64:
+        if (p.precision in f64_forms) != f64:
             continue
         if base == "rowpass" and p.nx == size and bool(p.thermal) == thermal:
             if thermal:
@@ -225,9 +230,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--precision", choices=("storage", "mixed"), default="storage",
-                    help="fused form timed as the value: FP32 fields with FP64 arithmetic (storage) or "
-                         "FP32 arithmetic (mixed); the other is reported beside it")
+    ap.add_argument("--precision", choices=("split", "storage", "mixed"), default="split",
+                    help="fused form timed as the value: FP32 fields with the row pass in FP32 and the column "
+                         "pass in FP64 arithmetic (split), FP64 arithmetic in both (storage) or FP32 in both "
+                         "(mixed); storage is reported beside split, split beside the others")
     ap.add_argument("--no-alt", action="store_true", help="skip the other fused form")
     ap.add_argument("--autotune", action="store_true",
                     help="before timing, run shg_autotune for every grid size of the workload (batch = the "
@@ -260,8 +266,10 @@ def main():
     from cushg_amd import abi, shg
 
     ps, desc = workload(args.workload)
-    prec = abi.PREC_FP32_STORAGE if args.precision == "storage" else abi.PREC_FP32_MIXED
-    alt_prec = abi.PREC_FP32_MIXED if prec == abi.PREC_FP32_STORAGE else abi.PREC_FP32_STORAGE
+    forms = {"split": abi.PREC_FP32_SPLIT, "storage": abi.PREC_FP32_STORAGE, "mixed": abi.PREC_FP32_MIXED}
+    names = {v: k for k, v in forms.items()}
+    prec = forms[args.precision]
+    alt_prec = abi.PREC_FP32_STORAGE if prec == abi.PREC_FP32_SPLIT else abi.PREC_FP32_SPLIT
     for p in ps:
         p.precision = prec
         if args.nz:
@@ -302,7 +310,7 @@ def main():
         t = time.perf_counter()
         ralt = run_once(h, qs)
         dalt = time.perf_counter() - t
-        alt = {"precision": "mixed" if alt_prec == abi.PREC_FP32_MIXED else "storage",
+        alt = {"precision": names[alt_prec],
                "value": round(pts_steps(qs, ralt) / dalt, 1), "unit": UNIT, "ms_per_step": round(dalt * 1e3, 3)}
         print(f"bench: other fused form {dalt:.2f} s", file=sys.stderr, flush=True)
     if not args.no_fp64:
@@ -357,17 +365,22 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == abi.PREC_FP32_STORAGE else "fp32",
-        "dtype_detail": ("fields fp32 in HBM (8 B per complex point); FFTs, propagator, phase and RK4 in "
-                         "f64 registers; reductions and heat residual f64, heat corrections fp32 GEMMs"
-                         if prec == abi.PREC_FP32_STORAGE else
-                         "fields, FFT butterflies, RK4 in fp32; twiddle/propagator products, dk*z phase, "
-                         "reductions and heat residual in f64"),
+        "dtype_detail": {
+            abi.PREC_FP32_STORAGE: "fields fp32 in HBM (8 B per complex point); FFTs, propagator, phase and RK4 in "
+                                   "f64 registers; reductions and heat residual f64, heat corrections fp32 GEMMs",
+            abi.PREC_FP32_SPLIT: "fields fp32 in HBM (8 B per complex point); row pass (row FFTs, RK4, heat "
+                                 "source) in fp32 with the dk*z phase in f64; column pass (column FFTs, "
+                                 "propagator) in f64 registers; reductions and heat residual f64, heat "
+                                 "corrections fp32 GEMMs",
+            abi.PREC_FP32_MIXED: "fields, FFT butterflies, RK4 in fp32; twiddle/propagator products, dk*z phase, "
+                                 "reductions and heat residual in f64"}[prec],
         "data": "synthetic (Gaussian pump from parameters)",
         "config": {"workload": f"{args.workload}: {desc}", "instances": len(ps),
                    "plans": {str(n): list(v) for n, v in plans.items()} if plans else "built-in",
-                   "form": "FP32 batched, " + ("FP32 fields / FP64 arithmetic (SHG_PREC_FP32_STORAGE)"
-                                               if prec == abi.PREC_FP32_STORAGE else
-                                               "FP32 arithmetic (SHG_PREC_FP32_MIXED)"),
+                   "form": "FP32 batched, " + {
+                       abi.PREC_FP32_STORAGE: "FP32 fields / FP64 arithmetic (SHG_PREC_FP32_STORAGE)",
+                       abi.PREC_FP32_SPLIT: "FP32 fields / FP32 row pass / FP64 column pass (SHG_PREC_FP32_SPLIT)",
+                       abi.PREC_FP32_MIXED: "FP32 arithmetic (SHG_PREC_FP32_MIXED)"}[prec],
                    "z_steps_per_run": int(sum(r.steps_done for r in res)),
                    "eta": [round(r.eta, 9) for r in res[:4]],
                    "peak_dT_K": [round(r.peak_dT_K, 6) for r in res[:4]],

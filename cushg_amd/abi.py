@@ -36,6 +36,7 @@ PREC_FP32_MIXED = 0
 PREC_FP64_PLAIN = 1
 PREC_FP32_STORAGE = 2
 PREC_FP64_FUSED = 3
+PREC_FP32_SPLIT = 4
 HEAT_JACOBI_REFERENCE = 0
 HEAT_DIRECT = 1
 # row-pass plan value of the wide form (include/shg.h SHG_PLAN_ROWS_WIDE)

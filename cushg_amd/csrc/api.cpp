@@ -164,7 +164,8 @@ int shg_validate(const shg_params* p) {
         return SHG_EINVAL;
     }
     if (p->precision != SHG_PREC_FP32_MIXED && p->precision != SHG_PREC_FP64_PLAIN &&
-        p->precision != SHG_PREC_FP32_STORAGE && p->precision != SHG_PREC_FP64_FUSED) {
+        p->precision != SHG_PREC_FP32_STORAGE && p->precision != SHG_PREC_FP64_FUSED &&
+        p->precision != SHG_PREC_FP32_SPLIT) {
         set_error("unknown precision %d", p->precision);
         return SHG_EINVAL;
     }

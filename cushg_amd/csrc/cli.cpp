@@ -120,7 +120,11 @@ void print_params(const shg_params& p) {
     std::printf("Crystal: %s, grid %d x %d x %d (heat z-slices %d), window %g x %g x %g um, THERMAL %s, %s\n",
                 p.crystal == SHG_CRYSTAL_MGO_PPLN ? "MgO:PPLN" : "MgO:sPPLT", p.nx, p.ny, p.nz,
                 p.nz_heat > 0 ? p.nz_heat : p.nz, p.lx_um, p.ly_um, p.lz_um, p.thermal ? "on" : "off",
-                p.precision == SHG_PREC_FP64_PLAIN ? "FP64 plain" : "FP32 mixed");
+                p.precision == SHG_PREC_FP64_PLAIN      ? "FP64 plain"
+                : p.precision == SHG_PREC_FP64_FUSED    ? "FP64 fused"
+                : p.precision == SHG_PREC_FP32_STORAGE  ? "FP32 storage"
+                : p.precision == SHG_PREC_FP32_SPLIT    ? "FP32 split"
+                                                        : "FP32 mixed");
 }
 
 void print_result(const shg_result& r, bool thermal) {

@@ -8,7 +8,7 @@
 // time (src/cuSHG.cu:11-15, src/headers/Crystal.h:26-52,
 // src/headers/Solver.h:32); here they are options:
 //   --nx= --ny= --nz= --nz-heat= --lx= --ly= --lz= (um)
-//   --crystal=spplt|ppln  --thermal=0|1  --precision=fp32|fp32-storage|fp64|fp64-fused
+//   --crystal=spplt|ppln  --thermal=0|1  --precision=fp32|fp32-storage|fp32-split|fp64|fp64-fused
 //   --heat=jacobi|direct  --heat-bc=open|oven  --scheme=reference|strang  --max-outer=
 //   --deff= (um/V) --period= (um) --alpha-p= --alpha-s= (1/um) --beta-s= (um/W)
 //   --rho-p= --rho-s= (rad) --waist-y=UM (elliptical pump; default circular)
@@ -81,9 +81,10 @@ int main(int argc, char** argv) {
         } else if (opt(a, "--thermal", &v)) {
             p.thermal = std::atoi(v);
         } else if (opt(a, "--precision", &v)) {
-            if (!pick(v, {"fp32", "fp32-storage", "fp64", "fp64-fused"},
-                      {SHG_PREC_FP32_MIXED, SHG_PREC_FP32_STORAGE, SHG_PREC_FP64_PLAIN, SHG_PREC_FP64_FUSED}, "--precision",
-                      p.precision))
+            if (!pick(v, {"fp32", "fp32-storage", "fp64", "fp64-fused", "fp32-split"},
+                      {SHG_PREC_FP32_MIXED, SHG_PREC_FP32_STORAGE, SHG_PREC_FP64_PLAIN, SHG_PREC_FP64_FUSED,
+                       SHG_PREC_FP32_SPLIT},
+                      "--precision", p.precision))
                 return 2;
         } else if (opt(a, "--heat", &v)) {
             if (!pick(v, {"jacobi", "direct"}, {SHG_HEAT_JACOBI_REFERENCE, SHG_HEAT_DIRECT}, "--heat", p.heat_solver))

@@ -10,7 +10,12 @@ enum class Arith {
     F32,  // FP32 fields, FP32 arithmetic          (SHG_PREC_FP32_MIXED)
     S32,  // FP32 fields, FP64 arithmetic          (SHG_PREC_FP32_STORAGE)
     F64,  // FP64 fields, FP64 arithmetic          (SHG_PREC_FP64_FUSED)
+    R32,  // FP32 fields, row pass in FP32 and column pass in FP64 arithmetic
+          // (SHG_PREC_FP32_SPLIT): the F32 row pass with the S32 column pass
 };
+// arithmetic of each pass of a form
+inline Arith row_arith(Arith a) { return a == Arith::R32 ? Arith::F32 : a; }
+inline Arith col_arith(Arith a) { return a == Arith::R32 ? Arith::S32 : a; }
 inline size_t arith_field_bytes(Arith a) { return a == Arith::F64 ? 16 : 8; }
 
 // One instance of a fused march, as the kernels see it.  Lives in device
@@ -70,8 +75,8 @@ hipError_t launch_colpass(int ny, Arith a, int cols_pw, dim3 grid, hipStream_t s
 // each in the nonlinear step; rows per workgroup fixed at max(1, 256/NX))
 constexpr int kRowsWide = SHG_PLAN_ROWS_WIDE;
 int rowpass_rows_per_wg(int nx, int rows);  // rows one workgroup covers under a plan value
-int rowpass_rows_default(int nx, Arith a);
-int rowpass_rows_auto(int nx, Arith a, long lines);  // by launch size
+int rowpass_rows_default(int nx, Arith a, bool thermal);
+int rowpass_rows_auto(int nx, Arith a, long lines, bool thermal);  // by launch size
 int colpass_cols_auto(int ny, Arith a, long lines);
 int colpass_cols_default(int ny, Arith a);
 bool colpass_cols_supported(int ny, int cols);

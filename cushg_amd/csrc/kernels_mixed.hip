@@ -17,15 +17,19 @@ hipError_t colpass_f64(int, int, dim3, hipStream_t, const MixedInst*, Step, cons
 // workgroup of the column pass, from shg_autotune runs on MI355X over
 // batches of 1, 4 and 16 instances per size (they agree with the tuned
 // plans at most sizes; shg_autotune refines them for a given workload).
-int rowpass_rows_default(int nx, Arith a) {
-    const bool fp64 = a != Arith::F32;
+// shg_autotune times the optics-only row pass; the THERMAL row pass at 512
+// in FP32 arithmetic, with its heat-voxel loads, runs faster at one row per
+// workgroup (2357 against 2526 ms per run of a 16 x 512 x 512 x 4000 THERMAL
+// batch).
+int rowpass_rows_default(int nx, Arith a, bool thermal) {
+    const bool fp64 = row_arith(a) != Arith::F32;
     switch (nx) {
         case 64: return fp64 ? 4 : 8;
         case 128: return fp64 ? 2 : 4;
         case 192: return 2;
         case 256: return 2;
         case 384: return 4;
-        case 512: return fp64 ? 1 : 2;
+        case 512: return fp64 || thermal ? 1 : 2;
     }
     return 0;
 }
@@ -36,9 +40,9 @@ int rowpass_rows_default(int nx, Arith a) {
 // NX/8 threads per row (108 us against 160 us).  Both forms give the same
 // bits (FMAs are contracted per source expression only), so the choice may
 // follow the launch size.
-int rowpass_rows_auto(int nx, Arith a, long lines) {
+int rowpass_rows_auto(int nx, Arith a, long lines, bool thermal) {
     if (lines > 0 && lines * (nx / 8) < 1024L * 64) return kRowsWide;
-    return rowpass_rows_default(nx, a);
+    return rowpass_rows_default(nx, a, thermal);
 }
 bool rowpass_rows_supported(int nx, int rows) {
     if (rows == kRowsWide) return nx >= 64 && nx <= 512;
@@ -49,7 +53,7 @@ int rowpass_rows_per_wg(int nx, int rows) {
     return rows;
 }
 int colpass_cols_default(int ny, Arith a) {
-    const bool fp64 = a != Arith::F32;
+    const bool fp64 = col_arith(a) != Arith::F32;
     switch (ny) {
         case 64: return 8;
         case 128: return 4;
@@ -74,20 +78,22 @@ bool colpass_cols_supported(int ny, int cols) {
 
 hipError_t launch_rowpass(int nx, bool thermal, Arith a, int rows, dim3 grid, hipStream_t st,
                           const MixedInst* insts, Step g, const double2* tw) {
-    switch (a) {
+    switch (row_arith(a)) {
         case Arith::F32: return rowpass_f32(nx, thermal, rows, grid, st, insts, g, tw);
         case Arith::S32: return rowpass_s32(nx, thermal, rows, grid, st, insts, g, tw);
         case Arith::F64: return rowpass_f64(nx, thermal, rows, grid, st, insts, g, tw);
+        default: break;
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_colpass(int ny, Arith a, int cols, dim3 grid, hipStream_t st, const MixedInst* insts, Step g,
                           const double2* tw) {
-    switch (a) {
+    switch (col_arith(a)) {
         case Arith::F32: return colpass_f32(ny, cols, grid, st, insts, g, tw);
         case Arith::S32: return colpass_s32(ny, cols, grid, st, insts, g, tw);
         case Arith::F64: return colpass_f64(ny, cols, grid, st, insts, g, tw);
+        default: break;
     }
     return hipErrorInvalidValue;
 }

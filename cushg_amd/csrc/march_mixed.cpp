@@ -25,29 +25,33 @@ Arith arith_of(int precision) {
     switch (precision) {
         case SHG_PREC_FP32_STORAGE: return Arith::S32;
         case SHG_PREC_FP64_FUSED: return Arith::F64;
+        case SHG_PREC_FP32_SPLIT: return Arith::R32;
         default: return Arith::F32;
     }
 }
 
+// profile names: the pass and the arithmetic it runs in
 static const char* arith_suffix(Arith a) {
     return a == Arith::F32 ? "" : a == Arith::S32 ? "_f64" : "_fp64fields";
 }
 const char* rowpass_name(int nx, bool th, Arith a) {
     static std::map<std::tuple<int, bool, int>, std::string> names;
+    a = row_arith(a);
     auto& s = names[{nx, th, (int)a}];
     if (s.empty()) s = std::string("rowpass_") + std::to_string(nx) + (th ? "_thermal" : "") + arith_suffix(a);
     return s.c_str();
 }
 const char* colpass_name(int ny, Arith a) {
     static std::map<std::pair<int, int>, std::string> names;
+    a = col_arith(a);
     auto& s = names[{ny, (int)a}];
     if (s.empty()) s = "colpass_" + std::to_string(ny) + arith_suffix(a);
     return s.c_str();
 }
 
-int plan_rows(shg_handle* h, int nx, Arith a, long lines) {
+int plan_rows(shg_handle* h, int nx, Arith a, long lines, bool thermal) {
     auto it = h->rows_plan.find({nx, (int)a});
-    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_auto(nx, a, lines);
+    return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_auto(nx, a, lines, thermal);
 }
 int plan_cols(shg_handle* h, int ny, Arith a, long lines) {
     auto it = h->cols_plan.find({ny, (int)a});
@@ -290,7 +294,8 @@ static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G, i
         for (auto* m : gr.m) maxn = std::max(maxn, row ? m->ny : m->nx);
         long lines = 0;
         for (auto* m : gr.m) lines += row ? m->ny : m->nx;
-        const int plan = row ? plan_rows(h, gr.key_n, gr.arith, lines) : plan_cols(h, gr.key_n, gr.arith, lines);
+        const int plan = row ? plan_rows(h, gr.key_n, gr.arith, lines, gr.thermal)
+                             : plan_cols(h, gr.key_n, gr.arith, lines);
         const int per = row ? rowpass_rows_per_wg(gr.key_n, plan) : plan;
         const dim3 grid((maxn + per - 1) / per, row ? 1 : 2, (unsigned)gr.m.size());
         const int ln = L.lane.at(&gr);
@@ -427,7 +432,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
                 lines += m->ny;
             }
             if (!cnt) continue;
-            const int rows = plan_rows(h, gr.key_n, gr.arith, lines);
+            const int rows = plan_rows(h, gr.key_n, gr.arith, lines, gr.thermal);
             const int per = rowpass_rows_per_wg(gr.key_n, rows);
             dim3 grid((maxny + per - 1) / per, 1, cnt);
             ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith), st);

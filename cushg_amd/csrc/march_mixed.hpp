@@ -45,7 +45,7 @@ const char* colpass_name(int ny, Arith a);
 // plans: the handle's set or tuned plan, else the built-in one for a launch
 // covering `lines` rows (row pass) or columns (column pass) in total (0: the
 // static default)
-int plan_rows(shg_handle* h, int nx, Arith a, long lines = 0);
+int plan_rows(shg_handle* h, int nx, Arith a, long lines = 0, bool thermal = false);
 int plan_cols(shg_handle* h, int ny, Arith a, long lines = 0);
 // sum of |A|^2 into a device slot, for either storage type
 int field_sum_sq(shg_handle* h, Arith a, const void* A, long n, double* slot);

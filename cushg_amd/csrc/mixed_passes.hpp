@@ -21,6 +21,8 @@
 //   (float2, double2)  SHG_PREC_FP32_STORAGE: everything between the FP32
 //                      load and the FP32 store in FP64
 //   (double2, double2) SHG_PREC_FP64_FUSED: FP64 fields and arithmetic
+// SHG_PREC_FP32_SPLIT runs the (float2, float2) row pass and the
+// (float2, double2) column pass.
 // The QPM phase dk*z is FP64 in all and reduced to a quarter turn before
 // the sin/cos series.
 #pragma once

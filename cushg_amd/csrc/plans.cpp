@@ -22,7 +22,8 @@ using namespace shg;
 namespace {
 
 bool fused_form(int precision) {
-    return precision == SHG_PREC_FP32_MIXED || precision == SHG_PREC_FP32_STORAGE || precision == SHG_PREC_FP64_FUSED;
+    return precision == SHG_PREC_FP32_MIXED || precision == SHG_PREC_FP32_STORAGE || precision == SHG_PREC_FP64_FUSED ||
+           precision == SHG_PREC_FP32_SPLIT;
 }
 
 // mean launch time [ms] of kernel class `name` over one march of `ps`, or

@@ -75,8 +75,20 @@ enum { SHG_SCHEME_REFERENCE = 0, SHG_SCHEME_STRANG = 1 };
  *                 operation between a load and a store -- FFTs, propagator,
  *                 phase, RK4 -- in FP64 registers.
  *   FP64_FUSED:   the fused, batched march with FP64 fields and FP64
- *                 arithmetic (twice the HBM bytes of the FP32-field forms). */
-enum { SHG_PREC_FP32_MIXED = 0, SHG_PREC_FP64_PLAIN = 1, SHG_PREC_FP32_STORAGE = 2, SHG_PREC_FP64_FUSED = 3 };
+ *                 arithmetic (twice the HBM bytes of the FP32-field forms).
+ *   FP32_SPLIT:   FP32 fields; the row pass (row transforms, RK4, heat
+ *                 source) in the FP32 arithmetic of FP32_MIXED, the column
+ *                 pass (column transforms, propagator) in the FP64
+ *                 arithmetic of FP32_STORAGE.  The column transforms carry
+ *                 most of FP32_MIXED's error, the row pass most of the
+ *                 FP64 arithmetic's cost. */
+enum {
+    SHG_PREC_FP32_MIXED = 0,
+    SHG_PREC_FP64_PLAIN = 1,
+    SHG_PREC_FP32_STORAGE = 2,
+    SHG_PREC_FP64_FUSED = 3,
+    SHG_PREC_FP32_SPLIT = 4
+};
 
 /* Heat solver for THERMAL runs.  JACOBI_REFERENCE: the reference's loop
  * (src/headers/Solver.h:332-356): Jacobi sweeps, convergence tested every
@@ -269,7 +281,7 @@ SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
 #define SHG_PLAN_ROWS_WIDE 16
 
 /* Launch plan of the fused passes for transform length n and precision
- * (SHG_PREC_FP32_MIXED, _FP32_STORAGE or _FP64_FUSED): rows per workgroup of
+ * (SHG_PREC_FP32_MIXED, _FP32_STORAGE, _FP64_FUSED or _FP32_SPLIT): rows per workgroup of
  * the row pass (1, 2, 4, 8 with n/8 threads per row holding 8 points each,
  * or SHG_PLAN_ROWS_WIDE) and columns per workgroup of the column pass
  * (2, 4, 8, 16); 0 keeps the current choice.  Replaces the reference's
@@ -278,8 +290,9 @@ SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
 SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t rows_per_wg, int32_t cols_per_wg);
 
 /* The plan the handle uses for (n, precision): tuned or set, else the
- * built-in plan for one n x n instance (built-in plans follow the launch
- * size: every plan gives the same results bit for bit). */
+ * built-in plan for one non-THERMAL n x n instance (built-in plans follow
+ * the launch size and, for the row pass, THERMAL: every plan gives the same
+ * results bit for bit). */
 SHG_API int shg_get_plan(shg_handle* h, int32_t n, int32_t precision, int32_t* rows_per_wg, int32_t* cols_per_wg);
 
 /* Autotuner: time every supported plan for transform length n on a

@@ -112,7 +112,7 @@ def _bad(**kw):
     (dict(power_W=float("nan")), ""),
     (dict(crystal=5), "crystal"),
     (dict(scheme=9), "scheme"),
-    (dict(precision=4), "precision"),
+    (dict(precision=5), "precision"),
     (dict(thermal=1, heat_solver=4), "heat"),
     (dict(thermal=1, heat_bc=5), "boundary"),
     (dict(Lambda_um=0.0), "Lambda"),

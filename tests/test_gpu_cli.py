@@ -39,7 +39,7 @@ def oracle_slices(p):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision,tol", [(abi.PREC_FP32_MIXED, 2e-5), (abi.PREC_FP32_STORAGE, 2e-6),
-                                           (abi.PREC_FP64_PLAIN, 1e-11)])
+                                           (abi.PREC_FP32_SPLIT, 2e-5), (abi.PREC_FP64_PLAIN, 1e-11)])
 def test_trace_slices_match_oracle_march(handle, precision, tol):
     p = small(precision)
     print(f"case trace precision={precision}", flush=True)

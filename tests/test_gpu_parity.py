@@ -1,11 +1,12 @@
 """GPU parity: the HIP library (through the C ABI) against the FP64 oracle,
-for the three fused forms.
+for the four fused forms.
 
 Tolerances: eta relative error <= 1e-5 against FP64 (the accuracy bar of
 the FP32 batched form) for the FP32-field forms; their fields are stored in
-FP32: relative L2 error <= 2e-5 with FP32 arithmetic (SHG_PREC_FP32_MIXED),
-<= 2e-6 with FP64 arithmetic (SHG_PREC_FP32_STORAGE, where only the stores
-round).  The FP64-field form (SHG_PREC_FP64_FUSED): 1e-10 for both.
+FP32: relative L2 error <= 2e-5 with FP32 arithmetic in the row pass
+(SHG_PREC_FP32_MIXED, SHG_PREC_FP32_SPLIT), <= 2e-6 with FP64 arithmetic
+throughout (SHG_PREC_FP32_STORAGE, where only the stores round).  The
+FP64-field form (SHG_PREC_FP64_FUSED): 1e-10 for both.
 """
 import numpy as np
 import pytest
@@ -27,11 +28,11 @@ def rel_l2(a, b):
 
 
 FORMS = [pytest.param(abi.PREC_FP32_MIXED, id="mixed"), pytest.param(abi.PREC_FP32_STORAGE, id="storage"),
-         pytest.param(abi.PREC_FP64_FUSED, id="fp64fused")]
+         pytest.param(abi.PREC_FP64_FUSED, id="fp64fused"), pytest.param(abi.PREC_FP32_SPLIT, id="split")]
 # (eta, field) relative tolerances per form: the FP64-field form differs
 # from the oracle only in FP64 rounding of a different operation order
 TOLS = {abi.PREC_FP32_MIXED: (ETA_TOL, FIELD_TOL), abi.PREC_FP32_STORAGE: (ETA_TOL, 2e-6),
-        abi.PREC_FP64_FUSED: (1e-10, 1e-10)}
+        abi.PREC_FP64_FUSED: (1e-10, 1e-10), abi.PREC_FP32_SPLIT: (ETA_TOL, FIELD_TOL)}
 
 
 @pytest.fixture(params=FORMS)

@@ -15,7 +15,7 @@ import pytest
 from cushg_amd import abi, configs
 from cushg_amd import shg as S
 
-FUSED = [abi.PREC_FP32_MIXED, abi.PREC_FP32_STORAGE, abi.PREC_FP64_FUSED]
+FUSED = [abi.PREC_FP32_MIXED, abi.PREC_FP32_STORAGE, abi.PREC_FP64_FUSED, abi.PREC_FP32_SPLIT]
 
 
 def uneven_batch(precision, thermal=False):
