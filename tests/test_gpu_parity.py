@@ -143,11 +143,12 @@ def test_poling_phase(handle, form):
 
 @pytest.mark.gpu
 def test_batch_equals_sequential(handle):
-    """Uneven batch (different grids, step counts and both fused forms) in
+    """Uneven batch (different grids, step counts and three fused forms) in
     one execution gives the same bits as running each instance alone."""
     ps = configs.sweep_batch(seed=3, n=6, scale_steps=0.02)
+    forms = (abi.PREC_FP32_MIXED, abi.PREC_FP32_STORAGE, abi.PREC_FP32_SPLIT)
     for i, p in enumerate(ps):
-        p.precision = abi.PREC_FP32_STORAGE if i % 2 else abi.PREC_FP32_MIXED
+        p.precision = forms[i % 3]
     rc, rb, ob = handle.run_batch(ps, fields=True)
     assert rc == 0
     for p, r, o in zip(ps, rb, ob):
