@@ -49,6 +49,25 @@ def _c_layout(tmp_path, struct, fields):
     return {ln.split()[0]: int(ln.split()[1]) for ln in out.splitlines()}
 
 
+def test_constants_match_header(tmp_path):
+    """Every enumerator and SHG_PLAN_ROWS_WIDE as a C compiler sees them
+    equal the binding's constants (status codes keep the SHG_ prefix in the
+    binding, the others drop it)."""
+    text = open(HEADER).read()
+    names = sorted(set(re.findall(r"\b(SHG_[A-Z0-9_]+)\s*=", text)) | {"SHG_PLAN_ROWS_WIDE"})
+    src = tmp_path / "consts.c"
+    body = "".join(f'  printf("{n} %d\\n", (int)({n}));\n' for n in names)
+    src.write_text(f'#include <stdio.h>\n#include "shg.h"\nint main(void){{\n{body}  return 0;\n}}\n')
+    exe = tmp_path / "consts"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    seen = {ln.split()[0]: int(ln.split()[1]) for ln in out.splitlines()}
+    assert len(seen) == len(names) and "SHG_PREC_FP32_SPLIT" in seen
+    for n, v in seen.items():
+        py = n if hasattr(abi, n) else n[len("SHG_"):]
+        assert getattr(abi, py) == v, n
+
+
 @pytest.mark.parametrize("struct,cls", [("shg_params", abi.ShgParams), ("shg_result", abi.ShgResult)])
 def test_struct_layout_matches_header(tmp_path, struct, cls):
     names = [f for f, _ in cls._fields_]
@@ -132,6 +151,14 @@ def test_validate_accepts_every_supported_size():
         assert L.shg_validate(C.byref(p)) == abi.SHG_OK
     for n in (0, 32, 96, 100, 320, 1024):
         assert L.shg_fft_size_supported(n) == 0
+
+
+def test_validate_accepts_every_precision():
+    L = shg.lib()
+    for prec in (abi.PREC_FP32_MIXED, abi.PREC_FP64_PLAIN, abi.PREC_FP32_STORAGE, abi.PREC_FP64_FUSED,
+                 abi.PREC_FP32_SPLIT):
+        p = _bad(precision=prec)
+        assert L.shg_validate(C.byref(p)) == abi.SHG_OK
 
 
 def test_null_arguments_are_einval_not_crashes():
