@@ -170,15 +170,26 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
         part_sq[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
 }
 
-// spectral division by lambda_x + lambda_y + lambda_z; grid (ceil(nxu/256), nyu, nz)
+// spectral division by lambda_x + lambda_y + lambda_z; grid (ceil(nxu/256),
+// nyu, z chunks), each thread walking the z range of its chunk.  FP32 data
+// are multiplied by the FP32 reciprocal of the FP64 sum (within an ulp or two
+// of the division, the precision of the FP32 transforms around it); FP64
+// data are divided.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx,
                                                    const double* __restrict__ ly, const double* __restrict__ lz,
-                                                   int nxu, int nyu) {
-    const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, z = blockIdx.z;
+                                                   int nxu, int nyu, int nz) {
+    const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
     if (x >= nxu) return;
-    const size_t i = ((size_t)z * nyu + j) * nxu + x;
-    v[i] = (R)((double)v[i] / (lx[x] + ly[j] + lz[z]));
+    const int per = (nz + gridDim.z - 1) / gridDim.z, z0 = blockIdx.z * per, z1 = min(nz, z0 + per);
+    const double lxy = lx[x] + ly[j];
+    for (int z = z0; z < z1; z++) {
+        const size_t i = ((size_t)z * nyu + j) * nxu + x;
+        if constexpr (sizeof(R) == 4)
+            v[i] = v[i] * __builtin_amdgcn_rcpf((float)(lxy + lz[z]));
+        else
+            v[i] = v[i] / (lxy + lz[z]);
+    }
 }
 
 // Even/odd fold of the symmetric axes of u[nz][nyu][nxu]: along a folded
@@ -380,8 +391,9 @@ template hipError_t launch_heat_resid<double>(hipStream_t, const HeatCoef&, cons
 
 template <typename R>
 hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz) {
-    hipLaunchKernelGGL((k_fd_divide<R>), dim3((nxu + 255) / 256, nyu, nz), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu,
-                       nyu);
+    const int zc = std::max(1, std::min(nz, 16));
+    hipLaunchKernelGGL((k_fd_divide<R>), dim3((nxu + 255) / 256, nyu, zc), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu,
+                       nyu, nz);
     return hipGetLastError();
 }
 template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int);
