@@ -50,6 +50,10 @@ const char* colpass_name(int ny, Arith a) {
 }
 
 int plan_rows(shg_handle* h, int nx, Arith a, long lines, bool thermal) {
+    if (thermal) {
+        auto th = h->rows_plan_th.find({nx, (int)a});
+        if (th != h->rows_plan_th.end() && th->second > 0) return th->second;
+    }
     auto it = h->rows_plan.find({nx, (int)a});
     return it != h->rows_plan.end() && it->second > 0 ? it->second : rowpass_rows_auto(nx, a, lines, thermal);
 }

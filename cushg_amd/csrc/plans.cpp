@@ -8,7 +8,9 @@
 // 16).  The radix sequence of the transform is
 // fixed by its length ({8, n/64, 8}).  The autotuner times every supported
 // candidate on a synthetic batched march of the caller's shape, with HIP
-// events around each launch, and keeps the fastest on the handle.
+// events around each launch, and keeps the fastest on the handle; the row
+// pass is timed a second time on a THERMAL march, whose voxel loads can
+// shift its best tile, and that choice is kept for THERMAL launches.
 #include <algorithm>
 #include <cfloat>
 #include <vector>
@@ -62,7 +64,10 @@ int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t r, int32_t
         return SHG_EINVAL;
     }
     const int a = (int)arith_of(precision);
-    if (r) h->rows_plan[{n, a}] = r;
+    if (r) {
+        h->rows_plan[{n, a}] = r;
+        h->rows_plan_th.erase({n, a});  // a set plan holds for THERMAL launches too
+    }
     if (c) h->cols_plan[{n, a}] = c;
     return SHG_OK;
 }
@@ -103,12 +108,14 @@ int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int
     const std::pair<int, int> key{n, (int)a};
     const auto rows_had = h->rows_plan.find(key) != h->rows_plan.end() ? h->rows_plan[key] : 0;
     const auto cols_had = h->cols_plan.find(key) != h->cols_plan.end() ? h->cols_plan[key] : 0;
+    const auto th_had = h->rows_plan_th.find(key) != h->rows_plan_th.end() ? h->rows_plan_th[key] : 0;
     auto restore = [&](int rc) {
         prof_reset(h);
         h->profiling = prof;
         if (rc != SHG_OK) {
             if (rows_had) h->rows_plan[key] = rows_had; else h->rows_plan.erase(key);
             if (cols_had) h->cols_plan[key] = cols_had; else h->cols_plan.erase(key);
+            if (th_had) h->rows_plan_th[key] = th_had; else h->rows_plan_th.erase(key);
         }
         return rc;
     };
@@ -130,6 +137,36 @@ int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int
         return restore(SHG_EDEVICE);
     }
     h->rows_plan[key] = best_rows;
+    // the THERMAL row pass: one thermo-optic pass with a 4-slice heat grid
+    // and the direct heat solver (the heat solve itself is not timed)
+    {
+        std::vector<shg_params> pt(ps);
+        for (auto& q : pt) {
+            q.thermal = 1;
+            q.nz_heat = std::min(4, nz);
+            q.heat_solver = SHG_HEAT_DIRECT;
+            q.max_outer = 1;
+        }
+        const char* name = rowpass_name(n, true, a);
+        h->rows_plan_th.erase(key);
+        if (time_pass(h, pt, name) < 0) return restore(SHG_EDEVICE);  // warm-up (heat tables)
+        double bt = DBL_MAX;
+        int br = 0;
+        for (int r : {1, 2, 4, 8, (int)SHG_PLAN_ROWS_WIDE}) {
+            if (!rowpass_rows_supported(n, r)) continue;
+            h->rows_plan_th[key] = r;
+            const double t = time_pass(h, pt, name);
+            if (t > 0 && t < bt) {
+                bt = t;
+                br = r;
+            }
+        }
+        if (!br) {
+            set_error("shg_autotune: no THERMAL row-pass plan ran at n=%d", n);
+            return restore(SHG_EDEVICE);
+        }
+        h->rows_plan_th[key] = br;
+    }
     best = DBL_MAX;
     for (int c : {2, 4, 8, 16}) {
         if (!colpass_cols_supported(n, c)) continue;

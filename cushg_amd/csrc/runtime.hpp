@@ -92,8 +92,9 @@ struct shg_handle {
     int n_partials = 0;
     // tuned launch plans: rows per workgroup (row pass, by (nx, form)),
     // columns per workgroup (column pass, by (ny, form)); absent = built-in
-    // default
-    std::map<std::pair<int, int>, int> rows_plan, cols_plan;
+    // default.  rows_plan_th: the autotuner's row plan for THERMAL row
+    // passes, ahead of rows_plan for those launches
+    std::map<std::pair<int, int>, int> rows_plan, cols_plan, rows_plan_th;
     // graph replay of the fused march: steps per captured graph (0: direct
     // launches; negative: automatic -- graphs of 32 steps for marches with
     // more than one independent launch lane), the device step counter and

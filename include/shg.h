@@ -284,7 +284,8 @@ SHG_API int shg_handle_set_profiling(shg_handle* h, int32_t on);
  * (SHG_PREC_FP32_MIXED, _FP32_STORAGE, _FP64_FUSED or _FP32_SPLIT): rows per workgroup of
  * the row pass (1, 2, 4, 8 with n/8 threads per row holding 8 points each,
  * or SHG_PLAN_ROWS_WIDE) and columns per workgroup of the column pass
- * (2, 4, 8, 16); 0 keeps the current choice.  Replaces the reference's
+ * (2, 4, 8, 16); 0 keeps the current choice.  A row plan set here holds
+ * for THERMAL and non-THERMAL launches alike.  Replaces the reference's
  * fixed 16x16 blocks (src/headers/Solver.h:119-135) and per-slice cuFFT
  * plans (src/headers/Solver.h:159,183). */
 SHG_API int shg_set_plan(shg_handle* h, int32_t n, int32_t precision, int32_t rows_per_wg, int32_t cols_per_wg);
@@ -298,7 +299,10 @@ SHG_API int shg_get_plan(shg_handle* h, int32_t n, int32_t precision, int32_t* r
 /* Autotuner: time every supported plan for transform length n on a
  * synthetic non-THERMAL march of `batch` n x n instances with nz slices
  * (HIP events around each launch) and keep the fastest row and column
- * plans on the handle.  No reference counterpart. */
+ * plans on the handle.  The row pass is timed again on a THERMAL march of
+ * the same shape (one thermo-optic pass, 4 heat slices) and that choice is
+ * kept for THERMAL launches (shg_get_plan reports the non-THERMAL one).
+ * No reference counterpart. */
 SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t batch, int32_t nz);
 
 /* Graph replay of the fused march: `steps` > 0 captures the launches of

@@ -85,17 +85,22 @@ def test_every_plan_gives_identical_results(precision, thermal):
 def test_autotune_picks_a_supported_plan_and_keeps_results():
     h = S.Handle(0)
     try:
-        for n, prec in ((128, abi.PREC_FP32_STORAGE), (256, abi.PREC_FP32_MIXED)):
+        for n, prec in ((128, abi.PREC_FP32_STORAGE), (256, abi.PREC_FP32_MIXED), (128, abi.PREC_FP32_SPLIT)):
             print(f"case autotune n={n} precision={prec}", flush=True)
             p = configs.scaled(configs.spplt64(), n=n, nz=24)
             p.precision = prec
-            ref = h.run(p)
+            pt = configs.scaled(configs.spplt256_thermal(), n=n, nz=24, nz_heat=6)
+            pt.precision = prec
+            ref, reft = h.run(p), h.run(pt, temperature=True)
             rows, cols = h.autotune(n, prec, batch=2, nz=16)
             print(f"  tuned plan rows={rows} cols={cols}")
             assert rows in (1, 2, 4, 8, abi.PLAN_ROWS_WIDE) and cols in (2, 4, 8, 16)
             assert h.get_plan(n, prec) == (rows, cols)
+            # every plan, tuned for optics or for THERMAL launches, gives the same bits
             r, out = h.run(p)
             same(([r], [out]), ([ref[0]], [ref[1]]))
+            rt, outt = h.run(pt, temperature=True)
+            same(([rt], [outt]), ([reft[0]], [reft[1]]))
         with pytest.raises(S.ShgError):
             h.autotune(100, abi.PREC_FP32_MIXED)
         with pytest.raises(S.ShgError):
