@@ -253,6 +253,9 @@ SHG_API int shg_run_batch(shg_handle* h, const shg_params* p, int32_t n, shg_res
  *   d_eta[2] = d eta / d T_C          (crystal temperature)
  *   d_eta[3 + s] = d eta / d poling_phase[s], s = 0..nz-1
  * d_eta must hold 3 + nz doubles.  checkpoint_every <= 0 selects a default.
+ * Arithmetic: FP64 for the FP64 forms; every FP32-field form is
+ * differentiated with FP32 fields and FP64 arithmetic (the FP32_STORAGE
+ * march), so they give the same result.
  * No reference counterpart (the reference computes no gradients). */
 SHG_API int shg_adjoint(shg_handle* h, const shg_params* p, int32_t checkpoint_every,
                 double* d_eta, shg_result* r);

@@ -139,6 +139,11 @@ def test_checkpoint_interval_does_not_change_the_gradient(handle):
     _, h0 = handle.adjoint(p, 1)
     _, h1 = handle.adjoint(p, 6)
     assert np.array_equal(h0, h1)
+    # every FP32-field form differentiates in FP64 arithmetic (FP32 storage)
+    for prec in (abi.PREC_FP32_MIXED, abi.PREC_FP32_SPLIT):
+        p.precision = prec
+        _, h2 = handle.adjoint(p, 6)
+        assert np.array_equal(h2, h0), prec
 
 
 @pytest.mark.gpu
