@@ -58,7 +58,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp, *objs, f"--offload-arch={ARCH}", "-fPIC", "-lrocblas"]
+    cmd = [HIPCC, "-shared", "-o", tmp, *objs, f"--offload-arch={ARCH}", "-fPIC", "-lrocblas", "-lhipblaslt"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}")
