@@ -81,6 +81,26 @@ def test_thermal_direct_vs_oracle(handle, nzh, form):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,bc", [(192, abi.HEAT_BC_OPEN_TOP), (384, abi.HEAT_BC_OPEN_TOP),
+                                  (192, abi.HEAT_BC_OVEN_SURROUNDED)])
+def test_thermal_direct_other_sizes_vs_oracle(handle, n, bc):
+    """The non-power-of-two grids: the heat solver's x fold at 192 (96 + 96) and 384, the z fold at 8 slices,
+    and the 3 x 64 / 6 x 64 transforms, against the oracle."""
+    p = small_thermal(n=n, nz=12, nzh=8)
+    p.heat_bc = bc
+    p.precision = abi.PREC_FP32_SPLIT
+    print(f"case thermal direct n={n} bc={bc}", flush=True)
+    ro, o = O.run(p, temperature=True)
+    rg, g = handle.run(p, temperature=True)
+    print(f"  eta gpu={rg.eta:.10e} cpu={ro.eta:.10e} rel={rel(rg.eta, ro.eta):.2e} "
+          f"dT {rg.peak_dT_K:.6e}/{ro.peak_dT_K:.6e} outer {rg.outer_iters}/{ro.outer_iters}")
+    assert rg.outer_iters == ro.outer_iters
+    assert rel(rg.eta, ro.eta) <= 1e-5
+    assert rel(rg.peak_dT_K, ro.peak_dT_K) <= 1e-5
+    assert np.abs(g["T"] - o["T"]).max() <= 1e-5 * ro.peak_dT_K
+
+
+@pytest.mark.gpu
 def test_thermal_jacobi_reference_loop_vs_oracle(handle):
     """The reference loop (sweeps + norm test every check_every) on both sides."""
     p = small_thermal(n=64, nz=16, nzh=6, solver=abi.HEAT_JACOBI_REFERENCE)
