@@ -18,7 +18,8 @@ z-steps, 50 mm crystal, 30 W, heat grid 512x512x400 -- over seeded
 
 The JSON line also carries:
   roofline        dominant kernel: algorithmic bytes per launch / average
-                  launch time from HIP events on the library's stream;
+                  launch time from HIP events that each launch's dispatch
+                  records (hipExtLaunchKernel) on the library's stream;
                   `traffic` = HBM bytes per launch from the committed
                   rocprofv3 counter summary (profiles/pmc_traffic.json).
   fp64_plain      the plain FP64 GPU form (one instance at a time) over
@@ -176,7 +177,8 @@ def kernel_roofline(h, ps):
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": "profiles/pmc_traffic.json (rocprofv3 TCC request counts + WRITE_SIZE)" if tr else None,
             "algorithmic_bytes_per_launch": round(total / n, 1), "avg_launch_us": round(ms / n * 1e3, 3),
-            "launches": n, "timer": "hipEvent on the library stream"}, kt
+            "launches": n, "timer": "HIP events recorded by each launch's own dispatch (hipExtLaunchKernel) on the "
+                                    "library stream"}, kt
 
 
 def cpu_baseline(ps, budget_s: float):

@@ -439,7 +439,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             const int rows = plan_rows(h, gr.key_n, gr.arith, lines, gr.thermal);
             const int per = rowpass_rows_per_wg(gr.key_n, rows);
             dim3 grid((maxny + per - 1) / per, 1, cnt);
-            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith), st);
+            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith), st, true);
             SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, st, gr.d, g, gr.tw));
         }
         for (auto* m : act)
@@ -460,7 +460,7 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
             if (!cnt) continue;
             const int cols = plan_cols(h, gr.key_n, gr.arith, lines);
             dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
-            ProfScope ps(h, colpass_name(gr.key_n, gr.arith), st);
+            ProfScope ps(h, colpass_name(gr.key_n, gr.arith), st, true);
             SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, st, gr.d, g, gr.tw));
         }
     }

@@ -539,14 +539,14 @@ static hipError_t rowpass_rows(int rows, dim3 grid, hipStream_t st, const MixedI
     dim3 block(NX / 8, rows);
     switch (rows) {
         case kRowsWide:
-            hipLaunchKernelGGL((k_rowpass_wide<NX, TH, S, C>), grid, dim3(NX * wide_rows<NX>()), 0, st, insts, g, tw);
+            pass_launch((k_rowpass_wide<NX, TH, S, C>), grid, dim3(NX * wide_rows<NX>()), st, insts, g, tw);
             break;
-        case 1: hipLaunchKernelGGL((k_rowpass<NX, TH, 1, S, C>), grid, block, 0, st, insts, g, tw); break;
-        case 2: hipLaunchKernelGGL((k_rowpass<NX, TH, 2, S, C>), grid, block, 0, st, insts, g, tw); break;
-        case 4: hipLaunchKernelGGL((k_rowpass<NX, TH, 4, S, C>), grid, block, 0, st, insts, g, tw); break;
+        case 1: pass_launch((k_rowpass<NX, TH, 1, S, C>), grid, block, st, insts, g, tw); break;
+        case 2: pass_launch((k_rowpass<NX, TH, 2, S, C>), grid, block, st, insts, g, tw); break;
+        case 4: pass_launch((k_rowpass<NX, TH, 4, S, C>), grid, block, st, insts, g, tw); break;
         case 8:
             if constexpr (NX / 8 * 8 <= 1024) {
-                hipLaunchKernelGGL((k_rowpass<NX, TH, 8, S, C>), grid, block, 0, st, insts, g, tw);
+                pass_launch((k_rowpass<NX, TH, 8, S, C>), grid, block, st, insts, g, tw);
                 break;
             }
             return hipErrorInvalidValue;
@@ -579,12 +579,12 @@ static hipError_t colpass_cols(int cols, dim3 grid, hipStream_t st, const MixedI
                                const double2* tw) {
     dim3 block(cols, NY / 8);
     switch (cols) {
-        case 2: hipLaunchKernelGGL((k_colpass<NY, 2, S, C>), grid, block, 0, st, insts, g, tw); break;
-        case 4: hipLaunchKernelGGL((k_colpass<NY, 4, S, C>), grid, block, 0, st, insts, g, tw); break;
-        case 8: hipLaunchKernelGGL((k_colpass<NY, 8, S, C>), grid, block, 0, st, insts, g, tw); break;
+        case 2: pass_launch((k_colpass<NY, 2, S, C>), grid, block, st, insts, g, tw); break;
+        case 4: pass_launch((k_colpass<NY, 4, S, C>), grid, block, st, insts, g, tw); break;
+        case 8: pass_launch((k_colpass<NY, 8, S, C>), grid, block, st, insts, g, tw); break;
         case 16:
             if constexpr (NY / 8 * 16 <= 1024 && sizeof(C) * 16 * FftShape<NY>::LDS <= 160 * 1024) {
-                hipLaunchKernelGGL((k_colpass<NY, 16, S, C>), grid, block, 0, st, insts, g, tw);
+                pass_launch((k_colpass<NY, 16, S, C>), grid, block, st, insts, g, tw);
                 break;
             }
             return hipErrorInvalidValue;

@@ -44,7 +44,13 @@ int Arena::reserve(size_t bytes) {
     return SHG_OK;
 }
 
-ProfScope::ProfScope(shg_handle* h_, const char* n, hipStream_t s) : h(h_), name(n), st(s ? s : h_->stream) {
+LaunchEvents& launch_events() {
+    thread_local LaunchEvents e;
+    return e;
+}
+
+ProfScope::ProfScope(shg_handle* h_, const char* n, hipStream_t s, bool one_launch)
+    : h(h_), name(n), st(s ? s : h_->stream), one(one_launch) {
     if (!h->profiling) return;
     if (h->event_pool.size() < 2) {
         for (int i = 0; i < 64; i++) {
@@ -57,12 +63,25 @@ ProfScope::ProfScope(shg_handle* h_, const char* n, hipStream_t s) : h(h_), name
     h->event_pool.pop_back();
     b = h->event_pool.back();
     h->event_pool.pop_back();
-    (void)hipEventRecord(a, st);
+    if (one)
+        launch_events() = LaunchEvents{a, b, false};
+    else
+        (void)hipEventRecord(a, st);
 }
 
 ProfScope::~ProfScope() {
     if (!a) return;
-    (void)hipEventRecord(b, st);
+    if (one) {
+        LaunchEvents& le = launch_events();
+        const bool used = le.used;
+        le = LaunchEvents{};
+        if (!used) {  // no launch took the events: an empty interval
+            (void)hipEventRecord(a, st);
+            (void)hipEventRecord(b, st);
+        }
+    } else {
+        (void)hipEventRecord(b, st);
+    }
     h->pending.push_back({name, {a, b}});
     if (h->pending.size() > 4096) prof_collect(h);
 }

@@ -116,12 +116,17 @@ struct shg_handle {
 namespace shg {
 
 // Profiling wrapper: brackets a launch with events on the handle's stream.
+// Times what runs on stream s while it lives, under `name` (when the handle
+// profiles).  With one_launch, the scope wraps exactly one fused-pass launch
+// and its events are recorded by that launch's own dispatch packet
+// (pass_launch), not as separate packets around it.
 struct ProfScope {
     shg_handle* h;
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t st;
-    ProfScope(shg_handle* h_, const char* n, hipStream_t s = nullptr);
+    bool one = false;
+    ProfScope(shg_handle* h_, const char* n, hipStream_t s = nullptr, bool one_launch = false);
     ~ProfScope();
 };
 void prof_collect(shg_handle* h);
