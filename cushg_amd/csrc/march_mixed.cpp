@@ -221,12 +221,46 @@ struct Group {
     int key_n;
     bool thermal;
     Arith arith;
+    int half = 0;  // interleave half: 1 runs its column pass half a step behind
     std::vector<MixedHost*> m;
     MixedInst* d = nullptr;
     const double2* tw = nullptr;
 };
-using RowGroups = std::map<std::tuple<int, bool, int>, Group>;
-using ColGroups = std::map<std::pair<int, int>, Group>;
+using RowGroups = std::map<std::tuple<int, bool, int, int>, Group>;
+using ColGroups = std::map<std::tuple<int, int, int>, Group>;
+
+// Interleaved halves.  The row pass is bound by vector-ALU issue and the
+// column pass by memory and LDS latency, so the two overlap well when they
+// run side by side.  Within one instance they cannot (each needs the
+// other's whole output), so a large class of like instances (same grid,
+// THERMAL flag and form) is split into two halves on two streams, and half
+// 1 runs its column pass of step g-1 beside half 0's row pass of step g:
+//   half 0:  R(g)   C(g)   R(g+1) C(g+1) ...
+//   half 1:  C(g-1) R(g)   C(g)   R(g+1) ...
+// Every instance still sees its own passes in order, and every pass of an
+// instance computes the same values in any launch that carries it, so the
+// results are those of one launch group.  A class is split when each half
+// holds at least kHalfMinPoints grid points; traced and profiled marches
+// (one stream) are not split.
+static constexpr size_t kHalfMinPoints = size_t(1) << 20;
+
+static void assign_halves(std::vector<MixedHost*>& act, bool allowed) {
+    std::map<std::tuple<int, int, bool, int>, std::vector<MixedHost*>> cls;
+    for (auto* m : act) {
+        m->half = 0;
+        cls[{m->nx, m->ny, m->thermal, (int)m->arith}].push_back(m);
+    }
+    if (!allowed) return;
+    for (auto& kv : cls) {
+        auto& v = kv.second;
+        size_t pts = 0;
+        for (auto* m : v) pts += m->N2;
+        if (v.size() < 2 || pts < 2 * kHalfMinPoints) continue;
+        // alternate down the nz order, so both halves march about as long
+        std::stable_sort(v.begin(), v.end(), [](const MixedHost* a, const MixedHost* b) { return a->nz > b->nz; });
+        for (size_t i = 0; i < v.size(); i++) v[i]->half = (int)(i & 1);
+    }
+}
 
 // Launch groups that share no instance, directly or through a chain of
 // groups, are independent: each such component runs on its own stream (up
@@ -248,7 +282,7 @@ static int march_lanes(shg_handle* h, RowGroups& rowg, ColGroups& colg, bool sin
     for (auto& kv : rowg)
         for (auto* m : kv.second.m) {
             const int a = find(idx[&kv.second]);
-            const int b = find(idx[&colg.at({m->ny, (int)m->arith})]);
+            const int b = find(idx[&colg.at({m->ny, (int)m->arith, m->half})]);
             par[a] = b;
         }
     std::map<int, int> comp;
@@ -306,7 +340,7 @@ static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G, i
         ls.push_back({row, &gr, plan, grid, ln});
         for (uint64_t v : {(uint64_t)row, (uint64_t)gr.key_n, (uint64_t)gr.thermal, (uint64_t)gr.arith,
                            (uint64_t)plan, (uint64_t)grid.x, (uint64_t)grid.z, (uint64_t)(uintptr_t)gr.d,
-                           (uint64_t)(uintptr_t)gr.tw, (uint64_t)ln})
+                           (uint64_t)(uintptr_t)gr.tw, (uint64_t)ln, (uint64_t)gr.half})
             key.push_back(v);
     };
     for (auto& kv : rowg) add(true, kv.second);
@@ -329,15 +363,22 @@ static int march_graph(shg_handle* h, RowGroups& rowg, ColGroups& colg, int G, i
             e = hipEventRecord(h->fork_ev, h->stream);
             for (int l = 1; l < L.n && e == hipSuccess; l++) e = hipStreamWaitEvent(L.streams[l], h->fork_ev, 0);
         }
+        // per step: half 0's row passes, half 1's column passes of the
+        // previous step (step -1 returns at once), half 0's column passes,
+        // half 1's row passes
+        const struct { bool row; int half, lag; } order[4] = {{true, 0, 0}, {false, 1, 1}, {false, 0, 0},
+                                                              {true, 1, 0}};
         for (int c = 0; c < C && e == hipSuccess; c++)
-            for (const Launch& l : ls) {
-                const Step st(h->step_counter, c);
-                hipStream_t s = L.streams[l.lane];
-                e = l.row ? launch_rowpass(l.gr->key_n, l.gr->thermal, l.gr->arith, l.plan, l.grid, s, l.gr->d, st,
-                                           l.gr->tw)
-                          : launch_colpass(l.gr->key_n, l.gr->arith, l.plan, l.grid, s, l.gr->d, st, l.gr->tw);
-                if (e != hipSuccess) break;
-            }
+            for (const auto& o : order)
+                for (const Launch& l : ls) {
+                    if (l.row != o.row || l.gr->half != o.half) continue;
+                    const Step st(h->step_counter, c - o.lag);
+                    hipStream_t s = L.streams[l.lane];
+                    e = l.row ? launch_rowpass(l.gr->key_n, l.gr->thermal, l.gr->arith, l.plan, l.grid, s, l.gr->d,
+                                               st, l.gr->tw)
+                              : launch_colpass(l.gr->key_n, l.gr->arith, l.plan, l.grid, s, l.gr->d, st, l.gr->tw);
+                    if (e != hipSuccess) break;
+                }
         for (int l = 1; l < L.n && e == hipSuccess; l++) {
             e = hipEventRecord(h->join_ev[l - 1], L.streams[l]);
             if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->join_ev[l - 1], 0);
@@ -364,16 +405,20 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
     size_t pool_used = 0;
     RowGroups rowg;
     ColGroups colg;
+    const bool traced = std::any_of(act.begin(), act.end(), [](const MixedHost* m) { return m->trace != nullptr; });
+    assign_halves(act, !traced && !h->profiling);
     for (auto* m : act) {
-        auto& rg = rowg[{m->nx, m->thermal, (int)m->arith}];
+        auto& rg = rowg[{m->nx, m->thermal, (int)m->arith, m->half}];
         rg.key_n = m->nx;
         rg.thermal = m->thermal;
         rg.arith = m->arith;
+        rg.half = m->half;
         rg.m.push_back(m);
-        auto& cg = colg[{m->ny, (int)m->arith}];
+        auto& cg = colg[{m->ny, (int)m->arith, m->half}];
         cg.key_n = m->ny;
         cg.thermal = false;
         cg.arith = m->arith;
+        cg.half = m->half;
         cg.m.push_back(m);
     }
     auto by_nz = [](const MixedHost* a, const MixedHost* b) { return a->nz > b->nz; };
@@ -403,7 +448,6 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         if (int rc = upload(kv.second)) return rc;
     int G = 0;
     for (auto* m : act) G = std::max(G, m->nz);
-    const bool traced = std::any_of(act.begin(), act.end(), [](const MixedHost* m) { return m->trace != nullptr; });
     Lanes L;
     if (int rc = march_lanes(h, rowg, colg, traced || h->profiling, L)) return rc;
     // graph replay: as set, or automatically (32 steps per graph) when the
@@ -423,46 +467,61 @@ static int march(shg_handle* h, std::vector<MixedHost*>& act, MixedInst* pool, s
         SHG_HIP(hipEventRecord(h->fork_ev, h->stream));
         for (int l = 1; l < nlanes; l++) SHG_HIP(hipStreamWaitEvent(streams[l], h->fork_ev, 0));
     }
-    for (int g = 0; g <= G; g++) {
-        for (auto& kv : rowg) {
-            Group& gr = kv.second;
-            hipStream_t st = streams[lane[&gr]];
-            int cnt = 0, maxny = 0;
-            long lines = 0;
-            for (auto* m : gr.m) {
-                if (m->nz < g) break;
-                cnt++;
-                maxny = std::max(maxny, m->ny);
-                lines += m->ny;
-            }
-            if (!cnt) continue;
-            const int rows = plan_rows(h, gr.key_n, gr.arith, lines, gr.thermal);
-            const int per = rowpass_rows_per_wg(gr.key_n, rows);
-            dim3 grid((maxny + per - 1) / per, 1, cnt);
-            ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith), st, true);
-            SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, st, gr.d, g, gr.tw));
+    auto row_pass = [&](Group& gr, int g) -> int {
+        hipStream_t st = streams[lane[&gr]];
+        int cnt = 0, maxny = 0;
+        long lines = 0;
+        for (auto* m : gr.m) {
+            if (m->nz < g) break;
+            cnt++;
+            maxny = std::max(maxny, m->ny);
+            lines += m->ny;
         }
+        if (!cnt) return SHG_OK;
+        const int rows = plan_rows(h, gr.key_n, gr.arith, lines, gr.thermal);
+        const int per = rowpass_rows_per_wg(gr.key_n, rows);
+        dim3 grid((maxny + per - 1) / per, 1, cnt);
+        ProfScope ps(h, rowpass_name(gr.key_n, gr.thermal, gr.arith), st, true);
+        SHG_HIP(launch_rowpass(gr.key_n, gr.thermal, gr.arith, rows, grid, st, gr.d, g, gr.tw));
+        return SHG_OK;
+    };
+    auto col_pass = [&](Group& gr, int g) -> int {
+        hipStream_t st = streams[lane[&gr]];
+        int cnt = 0, maxnx = 0;
+        long lines = 0;
+        for (auto* m : gr.m) {
+            if (m->nz <= g) break;
+            cnt++;
+            maxnx = std::max(maxnx, m->nx);
+            lines += m->nx;
+        }
+        if (!cnt) return SHG_OK;
+        const int cols = plan_cols(h, gr.key_n, gr.arith, lines);
+        dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
+        ProfScope ps(h, colpass_name(gr.key_n, gr.arith), st, true);
+        SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, st, gr.d, g, gr.tw));
+        return SHG_OK;
+    };
+    // per step: half 0's row passes, half 1's column passes of the previous
+    // step, half 0's column passes, half 1's row passes (assign_halves)
+    for (int g = 0; g <= G; g++) {
+        for (auto& kv : rowg)
+            if (kv.second.half == 0)
+                if (int rc = row_pass(kv.second, g)) return rc;
         for (auto* m : act)
             if (m->trace && m->trace->wants(g, m->nz))
                 if (int rc = deliver_slice(h, *m, g)) return rc;
-        if (g == G) break;
-        for (auto& kv : colg) {
-            Group& gr = kv.second;
-            hipStream_t st = streams[lane[&gr]];
-            int cnt = 0, maxnx = 0;
-            long lines = 0;
-            for (auto* m : gr.m) {
-                if (m->nz <= g) break;
-                cnt++;
-                maxnx = std::max(maxnx, m->nx);
-                lines += m->nx;
-            }
-            if (!cnt) continue;
-            const int cols = plan_cols(h, gr.key_n, gr.arith, lines);
-            dim3 grid((maxnx + cols - 1) / cols, 2, cnt);
-            ProfScope ps(h, colpass_name(gr.key_n, gr.arith), st, true);
-            SHG_HIP(launch_colpass(gr.key_n, gr.arith, cols, grid, st, gr.d, g, gr.tw));
-        }
+        if (g > 0)
+            for (auto& kv : colg)
+                if (kv.second.half == 1)
+                    if (int rc = col_pass(kv.second, g - 1)) return rc;
+        if (g < G)
+            for (auto& kv : colg)
+                if (kv.second.half == 0)
+                    if (int rc = col_pass(kv.second, g)) return rc;
+        for (auto& kv : rowg)
+            if (kv.second.half == 1)
+                if (int rc = row_pass(kv.second, g)) return rc;
     }
     for (int l = 1; l < nlanes; l++) {
         SHG_HIP(hipEventRecord(h->join_ev[l - 1], streams[l]));

@@ -32,6 +32,7 @@ struct MixedHost {
     MixedInst desc{};
     bool thermal = false;
     bool running = true;  // THERMAL: still in the outer loop
+    int half = 0;         // interleave half of the batched march (0 or 1)
     int outer = 0;
     long long steps = 0, sweeps = 0;
     double heat_res = 0;

@@ -474,7 +474,7 @@ void k_colpass(const MixedInst* __restrict__ insts, Step gs, const double2* __re
     constexpr int LD = FftShape<NY>::LDS;
     __shared__ C lds[COLS * LD];
     const MixedInst& I = insts[blockIdx.z];
-    if (g >= I.nz) return;
+    if (g < 0 || g >= I.nz) return;  // step -1: a lagging launch before the first step
     const int nx = I.nx;
     // A workgroup reads COLS*8 bytes of each 128-byte row segment; the K
     // column groups sharing a segment are placed on one XCD (workgroups are

@@ -229,7 +229,8 @@ class Handle:
 
     def set_graph_steps(self, steps: int):
         """Replay the fused march as hipGraphs of `steps` z-steps (0: direct
-        launches; -1, the default: graphs for batches with independent lanes)."""
+        launches; -1, the default: graphs for batches that march on several
+        streams)."""
         _check(lib().shg_set_graph_steps(self._h, steps))
 
 

@@ -243,7 +243,11 @@ SHG_API int shg_run_traced(shg_handle* h, const shg_params* p, int32_t every, sh
 /* n independent runs in one batched execution (replaces the one process
  * per point sweep of src/cuSHG.sh:58-94).  Instances may differ in every
  * parameter, including grid size and nz.  r[i].status holds each
- * instance's own status; the return value is the first failure or SHG_OK. */
+ * instance's own status; the return value is the first failure or SHG_OK.
+ * Like instances (same grid, THERMAL flag and precision) of at least 2^21
+ * grid points in total march as two halves on two streams, one half's
+ * column pass beside the other's row pass; every instance's result is
+ * bit-identical to its shg_run result either way. */
 SHG_API int shg_run_batch(shg_handle* h, const shg_params* p, int32_t n, shg_result* r);
 
 /* Gradient of eta (non-THERMAL runs) by a hand-written adjoint of the
@@ -312,8 +316,10 @@ SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t ba
  * that many consecutive z-steps (every launch group's row and column pass;
  * independent groups as parallel branches) once into a hipGraph, cached on
  * the handle, and replays it along the march; 0 launches every kernel
- * directly; -1 (the default) replays graphs of 32 steps for batches whose
- * launch groups form independent lanes and launches directly otherwise.
+ * directly; -1 (the default) replays graphs of 32 steps for batches that
+ * march on several streams (launch groups that share no instance, or the
+ * two halves of a large class of like instances, see shg_run_batch) and
+ * launches directly otherwise.
  * Runs with a per-slice trace or with profiling on always launch directly.  Replaces the reference's
  * per-slice host loop with its synchronisations (src/headers/Solver.h:276-308). */
 SHG_API int shg_set_graph_steps(shg_handle* h, int32_t steps);

@@ -159,6 +159,41 @@ def test_batch_equals_sequential(handle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("graph", [-1, 0], ids=["graph", "direct"])
+@pytest.mark.parametrize("thermal", [False, True], ids=["optics", "thermal"])
+@pytest.mark.parametrize("prec", [abi.PREC_FP32_SPLIT, abi.PREC_FP32_STORAGE], ids=["split", "storage"])
+def test_interleaved_halves_equal_sequential(handle, prec, thermal, graph):
+    """Eight like 512x512 instances (2^21 grid points) march as two
+    interleaved halves on two streams, one half's column pass of step g-1
+    beside the other's row pass of step g, by graph replay (the default) or
+    direct launches.  Step counts differ, so instances drop out of the
+    launches at different steps.  Every instance gets the bits it gets alone."""
+    ps = []
+    for i, p in enumerate(configs.ppln512_thermal_batch(8, seed=11)):
+        q = configs.scaled(p, nz=9 + 3 * i, nz_heat=4 if thermal else None)
+        q.thermal = 1 if thermal else 0
+        q.max_outer = 2
+        q.precision = prec
+        ps.append(q)
+    print(f"case interleaved halves prec={prec} thermal={thermal} graph={graph}", flush=True)
+    handle.set_graph_steps(graph)
+    try:
+        rc, rb, ob = handle.run_batch(ps, fields=True, temperature=thermal)
+    finally:
+        handle.set_graph_steps(-1)
+    assert rc == 0
+    for p, r, o in zip(ps, rb, ob):
+        r1, o1 = handle.run(p, temperature=thermal)
+        assert r.status == 0 and r.steps_done == r1.steps_done
+        assert r.eta == r1.eta
+        assert np.array_equal(o["pump"], o1["pump"])
+        assert np.array_equal(o["signal"], o1["signal"])
+        if thermal:
+            assert r.outer_iters == r1.outer_iters
+            assert np.array_equal(o["T"], o1["T"])
+
+
+@pytest.mark.gpu
 def test_invalid_size_is_einval(handle):
     p = configs.spplt64()
     p.nx = 100
