@@ -666,12 +666,10 @@ int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo&
 }
 
 int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc) {
-    const long n3 = (long)hs.nx * hs.ny * hs.nzh;
     {
         ProfScope ps(h, "heat_dk");
-        SHG_HIP(launch_heat_dk(h->stream, hs.T, hs.dk, hs.nn, n3, p->crystal, p->lambda_p_um, p->Lambda_um,
-                               p->therm_exp_a, p->therm_exp_b));
-        SHG_HIP(launch_heat_dkbase(h->stream, hs.dk, hs.dkbase, hs.nx * hs.ny, hs.nzh, p->nz));
+        SHG_HIP(launch_heat_dk(h->stream, hs.T, hs.dk, hs.nn, hs.dkbase, hs.nx * hs.ny, hs.nzh, p->nz, p->crystal,
+                               p->lambda_p_um, p->Lambda_um, p->therm_exp_a, p->therm_exp_b));
     }
     // temperature at the grid centre of heat slice 0 (src/headers/Efields.h:88,124)
     const size_t ic = (size_t)(hs.ny / 2) * hs.nx + hs.nx / 2;

@@ -53,9 +53,9 @@ hipError_t launch_fd_update(hipStream_t st, const HeatCoef& c, const HeatAxes& a
 hipError_t launch_reduce(hipStream_t st, const double* part, int n, bool is_max, double* out);
 hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny, int nz, double Tc, double Tpel,
                             bool oven);
-hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, long n, int crystal, double lp,
-                          double Lambda, double ea, double eb);
-hipError_t launch_heat_dkbase(hipStream_t st, const double* dk, double* base, int n2, int nzh, int nz);
+// n(T), dk(T) and the running-integral base per voxel, one pass over T
+hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, double* base, int n2, int nzh,
+                          int nz, int crystal, double lp, double Lambda, double ea, double eb);
 hipError_t launch_max(hipStream_t st, const double* a, long n, double* part, int nblocks);
 
 }  // namespace shg

@@ -316,32 +316,32 @@ __global__ void k_heat_init(double* __restrict__ T, float* __restrict__ Q, int n
 // src/headers/PhaseMatching.h:47-65) and (n_p(T), n_s(T)) per voxel for the
 // optics (the n(lambda, T) of src/headers/Solver.h:48,50 and
 // src/headers/Tfield.h:270), from one FP64 Sellmeier pair per voxel
-__global__ void k_heat_dk(const double* __restrict__ T, double* __restrict__ dk, float2* __restrict__ nn, long n,
-                          int crystal, double lp, double Lambda, double ea, double eb) {
+// Per voxel of a heat column: n_p(T), n_s(T) and dk(T) (src/headers/PhaseMatching.h:59,80),
+// and the base of the running dk integral per heat slice: the optical
+// slices s in [s0(h), s0(h) + cnt(h)) see  DKint_s = sum_{j<=s} dk[h(j)]
+// = dkbase[h] + s dk[h]  (src/headers/PhaseMatching.h:27-44 summed in O(1)).
+// One thread per (x, y) column walks the heat slices, so dk is written and
+// summed in one pass.
+__global__ __launch_bounds__(256) void k_heat_dk(const double* __restrict__ T, double* __restrict__ dk,
+                                                 float2* __restrict__ nn, double* __restrict__ base, int n2, int nzh,
+                                                 int nz, int crystal, double lp, double Lambda, double ea, double eb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
     const double ls = 0.5 * lp;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)blockDim.x * gridDim.x) {
-        const double t = T[i];
+    double cum = 0.0;
+    for (int h = 0; h < nzh; h++) {
+        const size_t v = (size_t)h * n2 + i;
+        const double t = T[v];
         const double np = refr_index<double>(crystal, lp, t);
         const double ns = refr_index<double>(crystal, ls, t);
         const double d = t - 25.0;
-        dk[i] = 2.0 * kPi * (2.0 * np / lp - ns / ls + 1.0 / (Lambda * (1.0 + ea * d + eb * d * d)));
-        nn[i] = make_float2((float)np, (float)ns);
-    }
-}
-
-// Base of the running dk integral per heat slice:  the optical slices
-// s in [s0(h), s0(h) + cnt(h)) see  DKint_s = sum_{j<=s} dk[h(j)]
-// = dkbase[h] + s dk[h]  (src/headers/PhaseMatching.h:27-44 summed in O(1)).
-__global__ void k_heat_dkbase(const double* __restrict__ dk, double* __restrict__ base, int n2, int nzh, int nz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n2) return;
-    double cum = 0.0;
-    for (int h = 0; h < nzh; h++) {
+        const double k = 2.0 * kPi * (2.0 * np / lp - ns / ls + 1.0 / (Lambda * (1.0 + ea * d + eb * d * d)));
+        dk[v] = k;
+        nn[v] = make_float2((float)np, (float)ns);
         const long long s0 = ((long long)h * nz + nzh - 1) / nzh;
         const long long s1 = ((long long)(h + 1) * nz + nzh - 1) / nzh;
-        const double d = dk[(size_t)h * n2 + i];
-        base[(size_t)h * n2 + i] = cum - (double)(s0 - 1) * d;
-        cum += (double)(s1 - s0) * d;
+        base[v] = cum - (double)(s0 - 1) * k;
+        cum += (double)(s1 - s0) * k;
     }
 }
 
@@ -439,14 +439,10 @@ hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny,
     return hipGetLastError();
 }
 
-hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, long n, int crystal, double lp,
-                          double Lambda, double ea, double eb) {
-    hipLaunchKernelGGL(k_heat_dk, dim3(4096), dim3(256), 0, st, T, dk, nn, n, crystal, lp, Lambda, ea, eb);
-    return hipGetLastError();
-}
-
-hipError_t launch_heat_dkbase(hipStream_t st, const double* dk, double* base, int n2, int nzh, int nz) {
-    hipLaunchKernelGGL(k_heat_dkbase, dim3((n2 + 255) / 256), dim3(256), 0, st, dk, base, n2, nzh, nz);
+hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, double* base, int n2, int nzh,
+                          int nz, int crystal, double lp, double Lambda, double ea, double eb) {
+    hipLaunchKernelGGL(k_heat_dk, dim3((n2 + 255) / 256), dim3(256), 0, st, T, dk, nn, base, n2, nzh, nz, crystal, lp,
+                       Lambda, ea, eb);
     return hipGetLastError();
 }
 
