@@ -276,7 +276,7 @@ def main():
         p.precision = prec
         if args.nz:
             if p.nz_heat > 0:  # keep the optical slices per heat slab
-                p.nz_heat = max(1, round(p.nz_heat * args.nz / p.nz))
+                p.nz_heat = min(args.nz, max(2, round(p.nz_heat * args.nz / p.nz)))
             p.nz = args.nz
         if args.max_outer:
             p.max_outer = args.max_outer

@@ -139,6 +139,12 @@ int shg_validate(const shg_params* p) {
         set_error("nz_heat=%d must be >= 2", nzh);
         return SHG_EINVAL;
     }
+    // optical slice s deposits its heat into slab floor(s nz_heat / nz): with
+    // more slabs than slices some slabs would get no source at all
+    if (p->thermal && nzh > p->nz) {
+        set_error("nz_heat=%d must not exceed nz=%d", nzh, p->nz);
+        return SHG_EINVAL;
+    }
     if (!(p->lx_um > 0) || !(p->ly_um > 0) || !(p->lz_um > 0)) {
         set_error("window/crystal lengths must be > 0");
         return SHG_EINVAL;

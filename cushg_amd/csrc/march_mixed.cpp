@@ -559,6 +559,11 @@ int run_mixed(shg_handle* h, const std::vector<const shg_params*>& ps, const std
     }
     const int rc = run_mixed_impl(h, ps, rs, ms);
     for (auto& m : ms) heat_release(m.heat);
+    // a failure that ends the whole batch leaves the instances it did not
+    // finish without results: each of them reports that failure
+    if (rc != SHG_OK)
+        for (size_t i = 0; i < ms.size(); i++)
+            if (!ms[i].finalised && rs[i]->status == SHG_OK) rs[i]->status = rc;
     return rc;
 }
 
@@ -730,6 +735,7 @@ static int run_mixed_impl(shg_handle* h, const std::vector<const shg_params*>& p
             if (int rc = heat_finish(h, p, m.heat, r)) return rc;
         }
         r->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        m.finalised = true;
     }
     return SHG_OK;
 }

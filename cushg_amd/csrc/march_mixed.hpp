@@ -32,6 +32,7 @@ struct MixedHost {
     MixedInst desc{};
     bool thermal = false;
     bool running = true;  // THERMAL: still in the outer loop
+    bool finalised = false;  // results written to *r (the last step of a run)
     int half = 0;         // interleave half of the batched march (0 or 1)
     int outer = 0;
     long long steps = 0, sweeps = 0;

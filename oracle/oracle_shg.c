@@ -866,6 +866,7 @@ int orc_march_heat(const shg_params* p, const double* T, double* Q, double* powe
     grid_t g;
     grid_init(p, &g);
     if (g.nzh < 2) return SHG_EINVAL;
+    if (p->thermal && g.nzh > g.nz) return SHG_EINVAL; /* a heat slab without an optical slice */
     const size_t N = (size_t)g.nx * g.ny, NV = N * g.nzh;
     fft_plan px, py;
     if (fft_plan_init(&px, g.nx) || fft_plan_init(&py, g.ny)) return SHG_EINVAL;
@@ -913,6 +914,7 @@ int orc_run(const shg_params* p, shg_result* r) {
     grid_t g;
     grid_init(p, &g);
     if (g.nzh < 2) return SHG_EINVAL;
+    if (p->thermal && g.nzh > g.nz) return SHG_EINVAL; /* a heat slab without an optical slice */
     const size_t N = (size_t)g.nx * g.ny, NV = N * g.nzh;
     fft_plan px, py;
     if (fft_plan_init(&px, g.nx) || fft_plan_init(&py, g.ny)) return SHG_EINVAL;
@@ -1028,7 +1030,7 @@ int orc_run(const shg_params* p, shg_result* r) {
                 steps++;
             }
             for (int h = 0; h < g.nzh; h++) {
-                if (!qcnt[h]) continue; /* no optical slice maps here: keep Q */
+                if (!qcnt[h]) continue; /* unreachable: nz_heat <= nz */
                 for (size_t i = 0; i < N; i++)
                     Q[(size_t)h * N + i] = qsum[(size_t)h * N + i] / qcnt[h];
             }

@@ -13,12 +13,20 @@ from an independent source:
                     module at 40 significant digits from the published
                     coefficients (Bruner et al. for MgO:sPPLT, as in
                     src/headers/Crystal.h:54-70).
-Usage: python tests/golden/make_golden.py
+  sellmeier_ppln.json
+                    the same for 5 mol% MgO-doped congruent LiNbO3
+                    (extraordinary ray): Gayer, Sacks, Galun, Arie,
+                    Appl. Phys. B 91, 343-348 (2008), Table 1, typed here
+                    from the paper; the grating expansion is Jundt's
+                    (Opt. Lett. 22, 1553 (1997)).  The reference has no
+                    PPLN entry (its README.md:52 names the crystal).
+Usage: python tests/golden/make_golden.py [fft|heat|sellmeier|ppln ...]
 """
 from __future__ import annotations
 
 import json
 import os
+import sys
 from decimal import Decimal, getcontext
 
 import numpy as np
@@ -178,8 +186,58 @@ def sellmeier():
         json.dump({"crystal": "MgO:sPPLT", "Lambda_um": 7.97, "rows": rows}, f, indent=1)
 
 
+def sellmeier_ppln():
+    getcontext().prec = 40
+    D = Decimal
+    # Gayer et al. 2008, Table 1, 5% MgO:CLN, n_e:
+    #   n_e^2 = a1 + b1 f + (a2 + b2 f)/(lambda^2 - (a3 + b3 f)^2)
+    #           + (a4 + b4 f)/(lambda^2 - a5^2) - a6 lambda^2,
+    #   f = (t - 24.5 C)(t + 570.82), lambda in um, t in C
+    a = {1: D("5.756"), 2: D("0.0983"), 3: D("0.2020"), 4: D("189.32"), 5: D("12.52"), 6: D("1.32e-2")}
+    b = {1: D("2.860e-6"), 2: D("4.700e-8"), 3: D("6.113e-8"), 4: D("1.516e-4")}
+
+    def n_e(lam, t):
+        t, lam = D(t), D(lam)
+        f = (t - D("24.5")) * (t + D("570.82"))
+        l2 = lam * lam
+        pole_uv = a[3] + b[3] * f
+        n2 = (a[1] + b[1] * f + (a[2] + b[2] * f) / (l2 - pole_uv * pole_uv)
+              + (a[4] + b[4] * f) / (l2 - a[5] * a[5]) - a[6] * l2)
+        return n2.sqrt()
+
+    pi = D("3.141592653589793238462643383279502884197")
+    # first-order grating of the crystal database entry and Jundt's thermal
+    # expansion of LiNbO3: L(T) = L(25) (1 + 1.54e-5 dT + 5.3e-9 dT^2)
+    Lam, ea, eb = D("6.97"), D("1.54e-5"), D("5.3e-9")
+
+    def dk(t):
+        lp, ls = D("1.064"), D("0.532")
+        d = D(t) - D(25)
+        return 2 * pi * (2 * n_e(lp, t) / lp - n_e(ls, t) / ls + 1 / (Lam * (1 + ea * d + eb * d * d)))
+
+    rows = []
+    for t in ("20", "25", "40", "60", "100", "150"):
+        rows.append({"T": float(t), "n_p": float(n_e("1.064", t)), "n_s": float(n_e("0.532", t)),
+                     "dk": float(dk(t))})
+    # other wavelengths of the fit's range (0.5-4 um) at 25 and 80 C
+    extra = []
+    for lam in ("0.5", "0.6328", "0.8", "1.55", "2.1", "3.0"):
+        for t in ("25", "80"):
+            extra.append({"lambda_um": float(lam), "T": float(t), "n": float(n_e(lam, t))})
+    with open(os.path.join(HERE, "sellmeier_ppln.json"), "w") as fh:
+        json.dump({"crystal": "MgO:PPLN (5% MgO:CLN, Gayer et al. 2008)", "Lambda_um": float(Lam),
+                   "therm_exp_a": float(ea), "therm_exp_b": float(eb), "rows": rows, "index": extra},
+                  fh, indent=1)
+
+
 if __name__ == "__main__":
-    fft_vectors()
-    heat_fv()
-    sellmeier()
+    parts = sys.argv[1:] or ["fft", "heat", "sellmeier", "ppln"]
+    if "fft" in parts:
+        fft_vectors()
+    if "heat" in parts:
+        heat_fv()
+    if "sellmeier" in parts:
+        sellmeier()
+    if "ppln" in parts:
+        sellmeier_ppln()
     print("fixtures written to", HERE)

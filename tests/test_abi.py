@@ -114,6 +114,25 @@ def test_crystal_database_matches_mirror_and_oracle(crystal):
     assert L.shg_params_set_crystal(C.byref(p), 7) == abi.SHG_EINVAL
 
 
+def test_library_ppln_matches_decimal_fixture():
+    """The library's own MgO:PPLN index and dk against the 40-digit decimal
+    evaluation of Gayer et al. (tests/golden/sellmeier_ppln.json)."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sellmeier_ppln.json")))
+    L = shg.lib()
+    p = abi.default_params()
+    assert L.shg_params_set_crystal(C.byref(p), abi.CRYSTAL_MGO_PPLN) == abi.SHG_OK
+    assert (p.Lambda_um, p.therm_exp_a, p.therm_exp_b) == (g["Lambda_um"], g["therm_exp_a"], g["therm_exp_b"])
+    for row in g["rows"]:
+        assert L.shg_refractive_index(abi.CRYSTAL_MGO_PPLN, 1.064, row["T"]) == pytest.approx(row["n_p"], rel=2e-15)
+        assert L.shg_refractive_index(abi.CRYSTAL_MGO_PPLN, 0.532, row["T"]) == pytest.approx(row["n_s"], rel=2e-15)
+        assert L.shg_delta_k(C.byref(p), row["T"]) == pytest.approx(row["dk"], rel=1e-11, abs=3e-14)
+    for row in g["index"]:
+        assert L.shg_refractive_index(abi.CRYSTAL_MGO_PPLN, row["lambda_um"], row["T"]) == pytest.approx(
+            row["n"], rel=2e-15)
+
+
 def _bad(**kw):
     p = abi.default_params()
     for k, v in kw.items():
@@ -126,6 +145,7 @@ def _bad(**kw):
     (dict(ny=1024), "ny=1024"),
     (dict(nz=1), "nz=1"),
     (dict(thermal=1, nz=100, nz_heat=1), "nz_heat=1"),
+    (dict(thermal=1, nz=10, nz_heat=20), "nz_heat=20 must not exceed nz=10"),
     (dict(lz_um=0.0), "lengths"),
     (dict(waist_um=-1.0), ""),
     (dict(power_W=float("nan")), ""),

@@ -89,6 +89,49 @@ def test_sellmeier_and_dk_decimal():
         assert O.delta_k(p, T) == pytest.approx(row["dk"], rel=1e-11, abs=1e-16)
 
 
+def _ppln_params():
+    p = abi.default_params()
+    abi.set_crystal(p, abi.CRYSTAL_MGO_PPLN)
+    return p
+
+
+def test_ppln_sellmeier_and_dk_decimal():
+    """MgO:PPLN against a 40-digit decimal evaluation of Gayer et al.'s
+    published n_e(lambda, T), coefficients typed afresh from the paper
+    (tests/golden/make_golden.py), and dk(T) with the database's period and
+    Jundt's expansion of the grating."""
+    g = json.load(open(os.path.join(GOLD, "sellmeier_ppln.json")))
+    p = _ppln_params()
+    assert (p.Lambda_um, p.therm_exp_a, p.therm_exp_b) == (g["Lambda_um"], g["therm_exp_a"], g["therm_exp_b"])
+    for row in g["rows"]:
+        T = row["T"]
+        assert O.refractive_index(abi.CRYSTAL_MGO_PPLN, 1.064, T) == pytest.approx(row["n_p"], rel=1e-15)
+        assert O.refractive_index(abi.CRYSTAL_MGO_PPLN, 0.532, T) == pytest.approx(row["n_s"], rel=1e-15)
+        # dk cancels terms of ~25 1/um: FP64 leaves ~1e-15 of that
+        assert O.delta_k(p, T) == pytest.approx(row["dk"], rel=1e-11, abs=3e-14)
+    for row in g["index"]:
+        assert O.refractive_index(abi.CRYSTAL_MGO_PPLN, row["lambda_um"], row["T"]) == pytest.approx(
+            row["n"], rel=1e-15)
+
+
+def test_ppln_against_published_magnitudes():
+    """Independent of the coefficient table: MgO:LN's extraordinary index at
+    1064 nm is about 2.148 near room temperature (congruent LN without MgO:
+    2.156), the 532 nm index about 2.22, and the first-order QPM period for
+    1064 -> 532 nm SHG in MgO:PPLN is about 6.9 um, shrinking as T rises.
+    A swapped pole or a dropped term moves these far outside the bounds."""
+    n1 = O.refractive_index(abi.CRYSTAL_MGO_PPLN, 1.064, 25.0)
+    n2 = O.refractive_index(abi.CRYSTAL_MGO_PPLN, 0.532, 25.0)
+    assert 2.140 < n1 < 2.156 and 2.215 < n2 < 2.235
+    periods = [1.064 / (2 * (O.refractive_index(abi.CRYSTAL_MGO_PPLN, 0.532, T)
+                             - O.refractive_index(abi.CRYSTAL_MGO_PPLN, 1.064, T))) for T in (25.0, 60.0, 100.0)]
+    assert all(6.8 < L < 7.0 for L in periods) and periods[0] > periods[1] > periods[2]
+    # normal dispersion: n falls with wavelength across the fit range
+    lams = np.linspace(0.5, 3.0, 26)
+    ns = [O.refractive_index(abi.CRYSTAL_MGO_PPLN, float(l), 40.0) for l in lams]
+    assert np.all(np.diff(ns) < 0)
+
+
 def test_dk_zero_near_56p92C():
     """dk vanishes at 56.9158 C for Lambda = 7.97 um (plane-wave phase matching)."""
     p = abi.default_params()
@@ -158,6 +201,156 @@ def test_free_space_gaussian_matches_analytic():
     assert np.abs(o["signal"]).max() == 0.0
     # diffraction is unitary: power kept to round-off
     assert r.P_pump_out_W == pytest.approx(r.P_pump_in_W, rel=1e-12)
+
+
+def test_walkoff_free_space_is_the_shifted_analytic_beam():
+    """Walk-off in free space: the propagator's linear phase
+    exp(-2 pi i fx dz tan(rho)) per slice shifts the beam along +x by
+    tan(rho) per unit length (shift theorem), so after nz-1 slices the saved
+    pump is the analytic Gaussian centred at x = L tan(rho).  A sign slip,
+    sin/rho for tan, a missing 2 pi or the wrong axis fails this."""
+    p = configs.spplt64()
+    p.nx = p.ny = 256
+    p.lx_um = p.ly_um = 1600.0
+    p.deff_um_per_V = 0.0
+    p.nz = 30
+    p.rho_p = 0.0042
+    r, o = O.run(p)
+    n = O.refractive_index(p.crystal, p.lambda_p_um, p.T_C)
+    w2 = p.waist_um ** 2
+    zR = math.pi * n * w2 / p.lambda_p_um
+    Ap0 = math.sqrt(4 * p.power_W / (EPS0 * C_LIGHT * math.pi * n * w2))
+    M = 1 + 1j * (p.lz_um - p.focal_um) / zR
+    dx, dy = p.lx_um / (p.nx - 1), p.ly_um / (p.ny - 1)
+    shift = p.lz_um * math.tan(p.rho_p)
+    assert shift > 10 * dx
+    x = (np.arange(p.nx) - p.nx / 2) * dx - shift
+    y = (np.arange(p.ny) - p.ny / 2) * dy
+    ref = (Ap0 / M) * np.exp(-(x[None, :] ** 2 + y[:, None] ** 2) / (w2 * M))
+    assert np.abs(o["pump"] - ref).max() < 1e-8 * np.abs(ref).max()
+    assert r.P_pump_out_W == pytest.approx(r.P_pump_in_W, rel=1e-12)
+
+
+def test_walkoff_of_both_fields_is_a_lattice_shift_of_the_coupled_march():
+    """With the same walk-off for pump and SH and tan(rho) dz equal to one
+    grid step, each slice's diffraction moves both fields by exactly one
+    sample along +x.  The nonlinear step is pointwise and the same at every x
+    (uniform dk), so the coupled march with walk-off equals the march without
+    it, rolled by nz-1 samples: this pins the SH walk-off term and its sign
+    with real conversion on."""
+    p = configs.spplt64()
+    p.nx = p.ny = 128
+    p.lx_um = p.ly_um = 1600.0
+    p.power_W = 20.0
+    p.nz = 24
+    dx = p.lx_um / (p.nx - 1)
+    dz = p.lz_um / (p.nz - 1)
+    r0, o0 = O.run(p)
+    q = p.copy()
+    q.rho_p = q.rho_s = math.atan(dx / dz)
+    r1, o1 = O.run(q)
+    assert r0.eta > 1e-4
+    for f in ("pump", "signal"):
+        ref = np.roll(o0[f], p.nz - 1, axis=1)
+        assert np.abs(o1[f] - ref).max() < 1e-10 * np.abs(ref).max(), f
+    assert r1.eta == pytest.approx(r0.eta, rel=1e-10)
+    # a walk-off of the SH alone is not a lattice shift: the fields separate
+    s = p.copy()
+    s.rho_s = q.rho_s
+    r2, _ = O.run(s)
+    assert abs(r2.eta - r0.eta) > 1e-3 * r0.eta
+
+
+def test_constant_poling_phase_rotates_the_sh_only():
+    """A phase phi on every slice enters the SH drive as exp(+i phi) and the
+    pump's back-conversion as exp(-i phi).  The map (P, S) -> (P, S e^{i phi})
+    then commutes with every step, so the march returns the pump unchanged
+    and the SH rotated by phi (with depletion and absorption on).  A wrong
+    sign on either field breaks the invariance."""
+    p = configs.spplt64()
+    p.power_W = 30.0
+    p.alpha_p, p.alpha_s, p.beta_s = 0.17e-6, 1.57e-6, 5e-5
+    p.nz = 40
+    r0, o0 = O.run(p)
+    assert r0.eta > 1e-3
+    phi = 0.7
+    q = p.copy()
+    q.set_poling(np.full(p.nz, phi))
+    r1, o1 = O.run(q)
+    sc = np.abs(o0["pump"]).max()
+    assert np.abs(o1["pump"] - o0["pump"]).max() < 1e-12 * sc
+    assert np.abs(o1["signal"] - np.exp(1j * phi) * o0["signal"]).max() < 1e-12 * sc
+    assert r1.eta == pytest.approx(r0.eta, rel=1e-12)
+
+
+def test_poling_phase_first_order_closed_form():
+    """Plane wave, small signal (undepleted pump, no loss): the SH after the
+    march is a sum of each slice's drive, to first order in the coupling.
+    Per RK4 call the reference scheme adds dz/6 (k1 + 2k2 + 2k3 + k4) with
+    k_j = i ks P0^2 exp(i(z_j dk + phi_s)) at z_j = s dz + {0, dz/4, dz/4,
+    dz/2}, both calls of slice s at the same z (src/headers/Solver.h:231-273),
+    and nz-1 slices precede the saved state.  Random per-slice phases make a
+    shifted slice index, a sign slip or a phase missing from one stage fail."""
+    p = _pw_params(31)
+    P0 = 1e-3
+    rng = np.random.default_rng(11)
+    phi = rng.uniform(-math.pi, math.pi, p.nz)
+    p.Lambda_um = 7.96  # off phase matching: dk dz is about 1.2 rad
+    p.set_poling(phi)
+    dz = p.lz_um / (p.nz - 1)
+    dk = O.delta_k(p, p.T_C)
+    assert abs(dk * dz) > 0.5
+    P = np.full((p.ny, p.nx), P0, np.complex128)
+    S = np.zeros_like(P)
+    for s in range(p.nz - 1):
+        P, S = O.ssfm_step(p, s, P, S)
+    dQ = 2 * p.deff_um_per_V / math.pi
+    n_s = O.refractive_index(0, 0.5 * p.lambda_p_um, p.T_C)
+    ks = 0.5 * math.pi * dQ / (n_s * 0.5 * p.lambda_p_um)
+    s_idx = np.arange(p.nz - 1)
+    e = np.exp(1j * (s_idx * dz * dk + phi[:-1]))
+    stages = 1 + 4 * np.exp(1j * dk * dz / 4) + np.exp(1j * dk * dz / 2)
+    S_ref = 2 * (dz / 6) * 1j * ks * P0 ** 2 * np.sum(e) * stages
+    assert abs(S[0, 0] - S_ref) < 1e-6 * abs(S_ref)
+    assert np.abs(S - S[0, 0]).max() < 1e-12 * abs(S_ref)
+    # the same march with the phases moved by one slice is far off
+    q = p.copy()
+    q.set_poling(np.roll(phi, 1))
+    P1 = np.full((p.ny, p.nx), P0, np.complex128)
+    S1 = np.zeros_like(P1)
+    for s in range(p.nz - 1):
+        P1, S1 = O.ssfm_step(q, s, P1, S1)
+    assert abs(S1[0, 0] - S_ref) > 0.1 * abs(S_ref)
+
+
+def test_poling_ramp_acts_as_a_dk_shift():
+    """A linear ramp phi_s = delta s dz adds delta to the phase mismatch at
+    the start of every slice; it differs from dk + delta only by delta times
+    the sub-step offsets (at most dz/2), so the ramp run sits far closer to the
+    dk + delta run than to the unramped or the dk - delta run."""
+    p = configs.spplt64()
+    p.nz = 60
+    dz = p.lz_um / (p.nz - 1)
+    delta = 2e-4
+    ramp = p.copy()
+    ramp.set_poling(delta * dz * np.arange(p.nz))
+    r_ramp, o_ramp = O.run(ramp)
+    r0, o0 = O.run(p)
+    # dk + delta through the grating period: 1/(Lambda' e) = 1/(Lambda e) + delta/(2 pi),
+    # e the grating's thermal expansion factor at T_C
+    d = p.T_C - 25.0
+    e = 1 + p.therm_exp_a * d + p.therm_exp_b * d * d
+    up = p.copy()
+    up.Lambda_um = 1.0 / (1.0 / p.Lambda_um + e * delta / (2 * math.pi))
+    r_up, o_up = O.run(up)
+    dn = p.copy()
+    dn.Lambda_um = 1.0 / (1.0 / p.Lambda_um - e * delta / (2 * math.pi))
+    r_dn, o_dn = O.run(dn)
+    assert O.delta_k(up, p.T_C) - O.delta_k(p, p.T_C) == pytest.approx(delta, rel=1e-9)
+    d_up = np.abs(o_ramp["signal"] - o_up["signal"]).max()
+    assert d_up < 0.05 * np.abs(o_ramp["signal"] - o0["signal"]).max()
+    assert d_up < 0.05 * np.abs(o_ramp["signal"] - o_dn["signal"]).max()
+    assert r_ramp.eta == pytest.approx(r_up.eta, rel=2e-2)
 
 
 def _astigmatic_gauss(p, Tc, zrel):
