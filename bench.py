@@ -90,6 +90,11 @@ def adjoint_cost(h) -> dict:
             "ratio": round(ta / tf, 3), "target": 3.0}
 
 
+# value form per workload when --precision is not given: the fastest form
+# whose eta is within 1e-5 of the FP64 plain form on every instance
+DEFAULT_FORM = {"sweep64": "storage"}
+
+
 def workload(name: str):
     from cushg_amd import configs
     if name == "ppln512t-b16":
@@ -159,24 +164,43 @@ def kernel_roofline(h, ps):
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}, kt
     total = 0.0
+    # the same launches priced at the minimum of the two-pass scheme: 32 B
+    # per point and pass for the fields, plus 24 B per point and z-step for
+    # THERMAL (T read, Q write, the FP64 running dk integral read + write),
+    # all of it charged to the row pass that does the thermal work
+    survey = 0.0
     for p, r in zip(ps, res):
         if (p.precision in f64_forms) != f64:
             continue
+        pts = p.nx * p.ny
         if base == "rowpass" and p.nx == size and bool(p.thermal) == thermal:
             if thermal:
                 total += r.outer_iters * rowpass_thermal_march_bytes(p)
+                survey += r.steps_done * pts * (32.0 + 24.0)
             else:  # nz + 1 row passes per march
-                total += (r.steps_done + r.outer_iters) * p.nx * p.ny * BYTES_PER_PT["rowpass"]
+                total += (r.steps_done + r.outer_iters) * pts * BYTES_PER_PT["rowpass"]
+                survey += r.steps_done * pts * 32.0
         elif base == "colpass" and p.ny == size:
-            total += r.steps_done * p.nx * p.ny * BYTES_PER_PT["colpass"]
+            total += r.steps_done * pts * BYTES_PER_PT["colpass"]
+            survey += r.steps_done * pts * 32.0
     ach = total / (ms * 1e-3) / 1e9
+    ach_s = survey / (ms * 1e-3) / 1e9
     tr = committed_traffic(name)
     alg = total / n
     traffic = round(alg * tr["traffic_over_algorithmic"], 1) if tr else None
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": "profiles/pmc_traffic.json (rocprofv3 TCC request counts + WRITE_SIZE)" if tr else None,
-            "algorithmic_bytes_per_launch": round(total / n, 1), "avg_launch_us": round(ms / n * 1e3, 3),
+            "traffic_source": ("not measured in this run: this run's algorithmic bytes per launch x the measured "
+                               "traffic / algorithmic ratio committed in profiles/pmc_traffic.json (rocprofv3 TCC "
+                               "request counts and WRITE_SIZE, on a 200-step, 20-slab cut of the workload)")
+                              if tr else None,
+            "algorithmic_bytes_per_launch": round(total / n, 1),
+            "algorithmic_bytes": "the layout's: fields 32 B/pt; THERMAL (n_p, n_s) float2, dk-base and dk doubles "
+                                 "read per launch and Q read-modify-written (about 66 B/pt per launch)",
+            "achieved_at_minimum_bytes": round(ach_s, 1), "frac_at_minimum_bytes": round(ach_s / HBM_PEAK_GBS, 4),
+            "minimum_bytes_per_launch": round(survey / n, 1),
+            "minimum_bytes": "32 B/pt per pass, + 24 B/pt per z-step for THERMAL (T 4, Q 4, FP64 dk integral 16)",
+            "avg_launch_us": round(ms / n * 1e3, 3),
             "launches": n, "timer": "HIP events recorded by each launch's own dispatch (hipExtLaunchKernel) on the "
                                     "library stream"}, kt
 
@@ -232,10 +256,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--precision", choices=("split", "storage", "mixed"), default="split",
+    ap.add_argument("--precision", choices=("split", "storage", "mixed"), default=None,
                     help="fused form timed as the value: FP32 fields with the row pass in FP32 and the column "
                          "pass in FP64 arithmetic (split), FP64 arithmetic in both (storage) or FP32 in both "
-                         "(mixed); storage is reported beside split, split beside the others")
+                         "(mixed); storage is reported beside split, split beside the others.  Default: the "
+                         "fastest form that holds every instance of the workload within the 1e-5 eta bar "
+                         "(storage for sweep64, whose far-off-phase-matching instances the split form's FP32 "
+                         "row transforms take to 1.15e-5; split otherwise)")
     ap.add_argument("--no-alt", action="store_true", help="skip the other fused form")
     ap.add_argument("--autotune", action="store_true",
                     help="before timing, run shg_autotune for every grid size of the workload (batch = the "
@@ -270,6 +297,8 @@ def main():
     ps, desc = workload(args.workload)
     forms = {"split": abi.PREC_FP32_SPLIT, "storage": abi.PREC_FP32_STORAGE, "mixed": abi.PREC_FP32_MIXED}
     names = {v: k for k, v in forms.items()}
+    if args.precision is None:
+        args.precision = DEFAULT_FORM.get(args.workload, "split")
     prec = forms[args.precision]
     alt_prec = abi.PREC_FP32_STORAGE if prec == abi.PREC_FP32_SPLIT else abi.PREC_FP32_SPLIT
     for p in ps:

@@ -215,7 +215,10 @@ static std::string geom_key(const shg_params* p, const HeatState& hs) {
     return b;
 }
 
-static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& hs) {
+// On failure returns nullptr with *rc = SHG_ENOCONV (eigen-solver) or
+// SHG_ENOMEM (device tables).
+static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& hs, int* rc) {
+    *rc = SHG_OK;
     const std::string key = geom_key(p, hs);
     auto it = h->heat_cache.find(key);
     if (it != h->heat_cache.end()) return it->second;
@@ -232,6 +235,7 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
                     axis_build(hs.nzh, hs.dz, true, true, false, false, hk, true, axes[2]);
     if (!ok) {
         set_error("heat: tridiagonal eigen-solver did not converge");
+        *rc = SHG_ENOCONV;
         return nullptr;
     }
     HeatGeom* g = new HeatGeom();
@@ -268,10 +272,12 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
         g->q32[i] = q32;
     }
     if (fail) {
+        (void)hipGetLastError();  // not left for a later launch check to report
         for (void* m : g->mem)
             if (m) (void)hipFree(m);
         delete g;
         set_error("heat: device allocation of solver tables failed");
+        *rc = SHG_ENOMEM;
         return nullptr;
     }
     h->heat_cache[key] = g;
@@ -357,8 +363,9 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
         set_error("arena exhausted (heat buffers)");
         return SHG_ENOMEM;
     }
-    HeatGeom* g = heat_geom(h, p, hs);
-    if (!g) return SHG_ENOCONV;
+    int grc = SHG_OK;
+    HeatGeom* g = heat_geom(h, p, hs, &grc);
+    if (!g) return grc;
     dt->ax = g->ax;
     for (int i = 0; i < 3; i++) {
         dt->fold[i] = g->fold[i];

@@ -36,6 +36,9 @@ int Arena::reserve(size_t bytes) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), want);
     if (e != hipSuccess) {
         base = nullptr;
+        // the failed allocation is also the thread's last HIP error: clear it,
+        // or the next launch check of a later call would report it again
+        (void)hipGetLastError();
         set_error("arena: hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
         return SHG_ENOMEM;
     }
