@@ -75,6 +75,52 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     return LIB
 
 
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer build (test
+# infrastructure): every source compiled with the sanitizers on its host
+# pass only (the GPU code objects are the normal ones), linked into a
+# separate library with the driver tests/native/host_sanity.cpp.
+SAN = os.path.join(HERE, "_san")
+SAN_LIB = os.path.join(SAN, "libshg_hip_san.so")
+SAN_EXE = os.path.join(SAN, "host_sanity")
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def _compile_san(src: str) -> str:
+    obj = os.path.join(SAN, os.path.basename(src) + ".o")
+    flags = [f for f in COMMON if f not in ("-O3",)] + ["-O1", "-g"]
+    cmd = [HIPCC, "-x", "hip", "-c", src, "-o", obj] + flags + SAN_HOST
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc (sanitize) failed for {src}:\n{r.stdout}")
+    return obj
+
+
+def build_sanitized(verbose: bool = False, jobs: int | None = None) -> str:
+    os.makedirs(SAN, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(8, os.cpu_count() or 4, len(srcs))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile_san, srcs))
+    cmd = [HIPCC, "-shared", "-o", SAN_LIB, *objs, f"--offload-arch={ARCH}", "-fPIC", "-lrocblas", "-lhipblaslt"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link (sanitize) failed:\n{r.stdout}")
+    drv = os.path.join(ROOT, "tests", "native", "host_sanity.cpp")
+    cmd = [HIPCC, "-O1", "-g", "-std=c++17", "-I", os.path.join(ROOT, "include"), drv, "-o", SAN_EXE,
+           "-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-L", SAN, "-lshg_hip_san",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitizer driver build failed:\n{r.stdout}")
+    if verbose:
+        print(f"built {SAN_EXE}")
+    return SAN_EXE
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize" in sys.argv:
+        build_sanitized(verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)
