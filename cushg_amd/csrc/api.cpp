@@ -224,6 +224,7 @@ int shg_handle_create(int32_t device, shg_handle** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
     for (int i = 0; i < 6 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&h->ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->d_scalars), sizeof(double) * 64);
+    if (e == hipSuccess && noalloc_attach(device) != SHG_OK) e = hipErrorInvalidSymbol;
     if (e != hipSuccess) {
         set_error("handle create: %s", hipGetErrorString(e));
         for (auto ev : h->ev)

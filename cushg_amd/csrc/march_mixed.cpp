@@ -90,6 +90,15 @@ int field_to_host(shg_handle* h, Arith a, const void* A, size_t n, double* out) 
 void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>& ex_s,
                         std::vector<double2>& ey_p, std::vector<double2>& ey_s,
                         std::vector<double2>& phase) {
+    ex_p.resize(m.nx);
+    ex_s.resize(m.nx);
+    ey_p.resize(m.ny);
+    ey_s.resize(m.ny);
+    phase.resize(m.thermal ? 0 : (size_t)m.nz * 6);
+    host_tables_into(m, ex_p.data(), ex_s.data(), ey_p.data(), ey_s.data(), m.thermal ? nullptr : phase.data());
+}
+
+void host_tables_into(MixedHost& m, double2* ex_p, double2* ex_s, double2* ey_p, double2* ey_s, double2* phase) {
     const shg_params* p = m.p;
     const int nx = m.nx, ny = m.ny, nz = m.nz;
     const double lp = p->lambda_p_um, ls = 0.5 * lp;
@@ -98,10 +107,6 @@ void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>&
     const double kwp = 2 * kPi * np / lp, kws = 2 * kPi * ns / ls;
     const double dfX = 1.0 / m.dx / nx, dfY = 1.0 / m.dy / ny;
     const double scale = 1.0 / ((double)nx * ny);
-    ex_p.resize(nx);
-    ex_s.resize(nx);
-    ey_p.resize(ny);
-    ey_s.resize(ny);
     for (int x = 0; x < nx; x++) {
         const int ic = (x + nx / 2) % nx;
         const double fx = (ic - 0.5 * nx) * dfX;
@@ -162,7 +167,6 @@ void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>&
     // non-THERMAL phase table: exp(i ((s + off) dz dk + poling[s])), FP64
     if (!m.thermal) {
         const double dk = delta_k(p->crystal, lp, p->Lambda_um, p->therm_exp_a, p->therm_exp_b, m.Tc);
-        phase.resize((size_t)nz * 6);
         for (int s = 0; s < nz; s++) {
             const double pol = p->poling_phase ? p->poling_phase[s] : 0.0;
             for (int c = 0; c < 2; c++)
@@ -171,8 +175,6 @@ void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>&
                     phase[(size_t)(s * 2 + c) * 3 + i] = make_double2(std::cos(ph), std::sin(ph));
                 }
         }
-    } else {
-        phase.clear();
     }
 }
 

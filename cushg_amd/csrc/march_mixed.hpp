@@ -56,6 +56,9 @@ int field_to_host(shg_handle* h, Arith a, const void* A, size_t n, double* out);
 // FP64 host tables (propagator factors, phase table, couplings) into m.desc
 void host_tables(MixedHost& m, std::vector<double2>& ex_p, std::vector<double2>& ex_s,
                  std::vector<double2>& ey_p, std::vector<double2>& ey_s, std::vector<double2>& phase);
+// the same tables into caller-provided arrays (nx, nx, ny, ny and, without
+// THERMAL, 6 nz entries; phase may be null with THERMAL)
+void host_tables_into(MixedHost& m, double2* ex_p, double2* ex_s, double2* ey_p, double2* ey_s, double2* phase);
 // uploads the tables, initialises the pump and a zero SH, reduces P_in into slots[0]
 int setup_fields(shg_handle* h, MixedHost& m);
 size_t mixed_bytes(const MixedHost& m);

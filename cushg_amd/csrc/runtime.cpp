@@ -124,15 +124,19 @@ static void tw_set(double2& t, double c, double s) {
     t.y = s;
 }
 
-// tw[m] = exp(-2 pi i m / n) in FP64
-template <typename C>
-static C* make_twiddles(int n) {
-    std::vector<C> t(n);
+void fill_twiddles64(double2* t, int n) {
     for (int k = 0; k < n; k++) {
         double c, s;
         tw_value(n, k, c, s);
         tw_set(t[k], c, s);
     }
+}
+
+// tw[m] = exp(-2 pi i m / n) in FP64
+template <typename C>
+static C* make_twiddles(int n) {
+    std::vector<C> t(n);
+    fill_twiddles64(t.data(), n);
     C* d = nullptr;
     if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(C) * n) != hipSuccess) return nullptr;
     if (hipMemcpy(d, t.data(), sizeof(C) * n, hipMemcpyHostToDevice) != hipSuccess) {

@@ -132,7 +132,13 @@ struct ProfScope {
 void prof_collect(shg_handle* h);
 void prof_reset(shg_handle* h);
 
+// the zero-allocation run's device scratch: looks up its address on
+// `device` (the current device) once; called by shg_handle_create
+int noalloc_attach(int device);
+
 const double2* twiddles64(shg_handle* h, int n);
+// the values of that table, tw[m] = exp(-2 pi i m / n), into host memory
+void fill_twiddles64(double2* t, int n);
 
 int sum_sq(shg_handle* h, const float2* A, long n, double* slot);
 int sum_sq64(shg_handle* h, const double2* A, long n, double* slot);

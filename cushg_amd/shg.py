@@ -30,7 +30,7 @@ EXPORTS = (
     "shg_refractive_index", "shg_delta_k", "shg_validate", "shg_last_error",
     "shg_handle_create", "shg_handle_destroy", "shg_run", "shg_run_traced", "shg_run_batch", "shg_adjoint",
     "shg_cli_main", "shg_last_kernel_times", "shg_handle_set_profiling", "shg_set_plan",
-    "shg_get_plan", "shg_autotune", "shg_set_graph_steps",
+    "shg_get_plan", "shg_autotune", "shg_set_graph_steps", "shg_run_noalloc",
 )
 
 _lib = None
@@ -82,6 +82,7 @@ def lib():
     L.shg_get_plan.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.shg_autotune.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.shg_set_graph_steps.argtypes = [H, C.c_int32]
+    L.shg_run_noalloc.argtypes = [H, P, R]
     _lib = L
     return L
 
@@ -152,6 +153,14 @@ class Handle:
         """shg_run: returns (ShgResult, dict of output arrays)."""
         r, out = self._prep_result(p, fields, temperature)
         _check(lib().shg_run(self._h, C.byref(p), C.byref(r)))
+        return r, out
+
+    def run_noalloc(self, p: ShgParams, fields: bool = True):
+        """shg_run_noalloc: one non-THERMAL fused-form run with every device
+        buffer in the library's fixed per-GPU scratch (no allocation).
+        Returns (ShgResult, dict of output arrays)."""
+        r, out = self._prep_result(p, fields, False)
+        _check(lib().shg_run_noalloc(self._h, C.byref(p), C.byref(r)))
         return r, out
 
     def run_traced(self, p: ShgParams, every: int, fn, fields: bool = False, temperature: bool = False):

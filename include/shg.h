@@ -324,6 +324,28 @@ SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t ba
  * per-slice host loop with its synchronisations (src/headers/Solver.h:276-308). */
 SHG_API int shg_set_graph_steps(shg_handle* h, int32_t steps);
 
+/* ---------------------------------------------------------------------
+ * Zero-allocation run.  No counterpart in the reference: this entry point
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * shg_run_noalloc is shg_run for one non-THERMAL instance of a fused form
+ * (SHG_PREC_FP32_SPLIT, _FP32_STORAGE, _FP32_MIXED, _FP64_FUSED) that calls
+ * no hipMalloc, hipFree or other allocating routine: every device buffer of
+ * the run (fields, saved output, propagator and phase tables, FFT twiddles,
+ * the instance descriptor, reduction slots, output staging) lives in a
+ * fixed device scratch array of SHG_NOALLOC_SCRATCH_MIB MiB that the
+ * library holds on each GPU it runs on until the process ends (attached by
+ * shg_handle_create).  The handle must exist (create it at set-up); the
+ * run uses the handle's stream and plans and gives shg_run's results bit
+ * for bit.  Limits: nz <= SHG_NOALLOC_MAX_NZ (SHG_EINVAL beyond), THERMAL
+ * and SHG_PREC_FP64_PLAIN return SHG_EUNSUPPORTED.  All calls on one GPU
+ * share that GPU's scratch, which keeps the last call's contents: calls
+ * that overlap on the same GPU (from several threads or handles) race. */
+#define SHG_NOALLOC_SCRATCH_MIB 20
+#define SHG_NOALLOC_MAX_NZ 16384
+SHG_API int shg_run_noalloc(shg_handle* h, const shg_params* p, shg_result* r);
+
 #ifdef __cplusplus
 }
 #endif
