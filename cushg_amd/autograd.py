@@ -1,10 +1,12 @@
 """torch.autograd.Function over the C ABI's shg_adjoint.
 
 ``eta(waist, focus, temperature, poling, params)`` returns the SH
-conversion efficiency of a non-THERMAL run as a 0-dim float64 tensor whose
-backward pass applies the gradient computed by the library's hand-written
-adjoint of the discrete SSFM/RK4 march (checkpointed recomputation; no
-reference counterpart -- the reference computes no gradients).  The forward
+conversion efficiency of a run as a 0-dim float64 tensor whose backward
+pass applies the gradient computed by the library's hand-written adjoint of
+the discrete SSFM/RK4 march (checkpointed recomputation; with THERMAL, at
+the converged thermo-optic steady state; no reference counterpart -- the
+reference computes no gradients).  `temperature` moves T_C and T_peltier_C
+together (the reference's argv[3]).  The forward
 and the adjoint run together in the forward call, so ``backward`` only
 scales the stored gradient.  PyTorch is plumbing here: the inputs are read
 into FP64 host scalars / a host poling array and passed through the C ABI.
@@ -48,6 +50,6 @@ class EtaFunction(torch.autograd.Function):
 def eta(waist, focus, temperature, poling, params: ShgParams, handle: shg.Handle | None = None,
         checkpoint_every: int = 0):
     """Differentiable eta(waist [um], focus [um], T [C], poling [rad, nz]) of
-    the non-THERMAL run described by `params` (its other fields are kept)."""
+    the run described by `params` (its other fields are kept)."""
     h = handle if handle is not None else shg.Handle(0)
     return EtaFunction.apply(waist, focus, temperature, poling, params, h, checkpoint_every)

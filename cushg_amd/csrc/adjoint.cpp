@@ -75,11 +75,10 @@ double dn_dT(int crystal, double lam, double T) {
 
 }  // namespace
 
+int run_adjoint_thermal(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r);
+
 int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_result* r) {
-    if (p->thermal) {
-        set_error("shg_adjoint: gradients of THERMAL runs are not provided (the heat solve is not differentiated)");
-        return SHG_EUNSUPPORTED;
-    }
+    if (p->thermal) return run_adjoint_thermal(h, p, ck, d_eta, r);
     if (p->waist_y_um > 0) {
         set_error("shg_adjoint: the gradient is taken for the circular pump (waist_y_um = 0)");
         return SHG_EUNSUPPORTED;

@@ -672,6 +672,43 @@ int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo&
     return dt.fp64 ? direct_solve<double>(h, p, c, hs, dt, info) : direct_solve<float>(h, p, c, hs, dt, info);
 }
 
+int heat_adjoint_solve(shg_handle* h, const shg_params* p, HeatState& hs, const double* mu, double* nu, double* w,
+                       double* u64, double* v64, double* tp) {
+    const HeatCoef c = heat_coef(p, hs.nzh);
+    int grc = SHG_OK;
+    HeatGeom* g = heat_geom(h, p, hs, &grc);
+    if (!g) return grc;
+    if (!g->q64[0] || !g->q64[1] || !g->q64[2]) {
+        set_error("heat adjoint: FP64 eigenbases missing");
+        return SHG_ENOMEM;
+    }
+    const int nx = c.nxu, nyu = c.nyu, nz = hs.nzh;
+    const bool fx = g->fold[0], fz = g->fold[2];
+    double* a = u64;
+    double* b = v64;
+    ProfScope ps(h, "heat_adjoint_solve");
+    SHG_HIP(launch_adj_heat_in(h->stream, c, g->ax, mu, a));
+    if (fx || fz) {
+        SHG_HIP(launch_fd_fold<double>(h->stream, a, b, nx, nyu, nz, fx, fz));
+        std::swap(a, b);
+    }
+    // nu = W^-1/2 Q S^-1 Q^T W^-1/2 mu: the exact inverse, in FP64 (the
+    // operator is symmetric, so the transposed solve is this one)
+    if (int rc = axis_gemm<double>(h, 0, true, fx, g->q64[0], a, b, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm<double>(h, 1, true, false, g->q64[1], b, a, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm<double>(h, 2, true, fz, g->q64[2], a, b, nx, nyu, nz)) return rc;
+    SHG_HIP(launch_fd_divide<double>(h->stream, b, g->ax, nx, nyu, nz));
+    if (int rc = axis_gemm<double>(h, 2, false, fz, g->q64[2], b, a, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm<double>(h, 1, false, false, g->q64[1], a, b, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm<double>(h, 0, false, fx, g->q64[0], b, a, nx, nyu, nz)) return rc;
+    SHG_HIP(hipMemsetAsync(nu, 0, sizeof(double) * (size_t)c.nx * c.ny * nz, h->stream));
+    int nb = 0;
+    SHG_HIP(launch_fd_update<double>(h->stream, c, g->ax, a, nu, hs.part, fx, fz, &nb));
+    SHG_HIP(launch_adj_heat_out(h->stream, c, g->ax, mu, nu, w, hs.part));
+    SHG_HIP(launch_reduce(h->stream, hs.part, heat_grid_blocks(c), false, tp));
+    return SHG_OK;
+}
+
 int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc) {
     {
         ProfScope ps(h, "heat_dk");

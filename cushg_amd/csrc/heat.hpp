@@ -41,6 +41,13 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs);
 int heat_init(shg_handle* h, const shg_params* p, HeatState& hs);
 int heat_solve(shg_handle* h, const shg_params* p, HeatState& hs, HeatSolveInfo& info);
 int heat_prepare_optics(shg_handle* h, const shg_params* p, HeatState& hs, double* Tc);
+// Adjoint of the steady state (DIRECT solver tables): nu = A^-1 mu on the
+// unknown nodes (0 on the oven nodes) by one FP64 fast-diagonalisation
+// solve, w = dr/dQ nu (the heat-source cotangent, 0 where Q is not used) and
+// *tp (device) = the derivative of <mu, T> with the oven temperature.  mu,
+// nu, w: [nzh][ny][nx]; u64, v64: nxu * nyu * nzh doubles of work space.
+int heat_adjoint_solve(shg_handle* h, const shg_params* p, HeatState& hs, const double* mu, double* nu, double* w,
+                       double* u64, double* v64, double* tp);
 int heat_finish(shg_handle* h, const shg_params* p, HeatState& hs, shg_result* r);
 void heat_release(HeatState& hs);
 

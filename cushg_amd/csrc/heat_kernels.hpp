@@ -57,5 +57,16 @@ hipError_t launch_heat_init(hipStream_t st, double* T, float* Q, int nx, int ny,
 hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* nn, double* base, int n2, int nzh,
                           int nz, int crystal, double lp, double Lambda, double ea, double eb);
 hipError_t launch_max(hipStream_t st, const double* a, long n, double* part, int nblocks);
+// adjoint of the steady state: u = W^-1/2 mu on the unknown block; after
+// nu = A^-1 mu, the heat-source cotangent w and the partial sums of the oven
+// temperature's derivative (at most heat_grid_blocks(c)); dL/dT per voxel
+// from the adjoint row passes' accumulators (into gnp), plus tc at the
+// centre voxel of slab 0
+hipError_t launch_adj_heat_in(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu, double* u);
+hipError_t launch_adj_heat_out(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu,
+                               const double* nu, double* w, double* part);
+hipError_t launch_adj_heat_grad(hipStream_t st, const double* T, double* gnp, const double* gns, const double* gA,
+                                const double* gC, int nx, int ny, int nzh, int nz, int crystal, double lp,
+                                double Lambda, double ea, double eb, double tc);
 
 }  // namespace shg

@@ -359,6 +359,108 @@ __global__ __launch_bounds__(256) void k_max(const double* __restrict__ a, long 
 }
 
 // ------------------------------------------------------------------
+// adjoint of the steady state (THERMAL gradients)
+// ------------------------------------------------------------------
+__device__ __forceinline__ bool heat_fixed(const HeatCoef& c, int x, int y) {
+    return y == 0 || (c.oven && (x == 0 || x == c.nx - 1 || y == c.ny - 1));
+}
+
+// u = W^-1/2 mu on the unknown block, in the layout k_heat_resid writes:
+// the right-hand side of the transposed (= the same, symmetric) solve.
+__global__ __launch_bounds__(256) void k_adj_heat_in(HeatCoef c, HeatAxes a, const double* __restrict__ mu,
+                                                     double* __restrict__ u) {
+    const int xu = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    if (xu >= c.nxu || j >= c.nyu) return;
+    const int x = xu + c.x0, y = j + 1;
+    const ZRange zr = zchunk(c);
+    const double wxy = a.wx[xu] * a.wy[j];
+    for (int z = zr.z0; z < zr.z1; z++)
+        u[((size_t)z * c.nyu + j) * c.nxu + xu] =
+            mu[((size_t)z * c.ny + y) * c.nx + x] * rsqrt(wxy * a.wz[z]);
+}
+
+// After nu = A^-1 mu (A the finite-volume operator of the steady state):
+//   w_v = dr_v/dQ_v nu_v = A_vv L2 / k nu_v at the inner nodes that take Q
+//   (the boundary nodes omit it, src/headers/Tfield.h:119-196), 0 elsewhere;
+//   partial sums of  sum_{oven nodes} mu_v + sum_{unknowns} nu_v dr_v/dT_pel,
+//   the derivative of <mu, T> with the oven temperature (dr_v/dT_pel: the
+//   coupling of node v to its oven neighbours, A_vv (F(T) - T) at T = 1 on
+//   the oven nodes, 0 elsewhere, without Q and T_inf).
+__global__ __launch_bounds__(256) void k_adj_heat_out(HeatCoef c, HeatAxes a, const double* __restrict__ mu,
+                                                      const double* __restrict__ nu, double* __restrict__ w,
+                                                      double* __restrict__ part) {
+    __shared__ double red[4];
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const ZRange zr = zchunk(c);
+    double acc = 0.0;
+    if (x < c.nx && y < c.ny) {
+        const bool fixed = heat_fixed(c, x, y);
+        const int xu = x - c.x0, j = y - 1;
+        HeatCoef c0 = c;
+        c0.Tinf = 0.0;
+        Nb t;
+        t.c = 0.0;
+        t.xm = x > 0 && heat_fixed(c, x - 1, y) ? 1.0 : 0.0;
+        t.xp = x < c.nx - 1 && heat_fixed(c, x + 1, y) ? 1.0 : 0.0;
+        t.ym = y > 0 && heat_fixed(c, x, y - 1) ? 1.0 : 0.0;
+        t.yp = y < c.ny - 1 && heat_fixed(c, x, y + 1) ? 1.0 : 0.0;
+        t.zm = 0.0;
+        t.zp = 0.0;
+        const bool inner_xy = x > 0 && x < c.nx - 1 && y > 0 && y < c.ny - 1;
+        for (int z = zr.z0; z < zr.z1; z++) {
+            const size_t i = ((size_t)z * c.ny + y) * c.nx + x;
+            if (fixed) {
+                acc += mu[i];
+                w[i] = 0.0;
+                continue;
+            }
+            const double wz = a.wz[z];
+            const double aii = a.kx[xu] * a.wy[j] * wz + a.wx[xu] * a.ky[j] * wz + a.wx[xu] * a.wy[j] * a.kz[z];
+            const bool takes_q = inner_xy && z > 0 && z < c.nz - 1;
+            w[i] = takes_q ? aii * c.L2 * c.ikth * nu[i] : 0.0;
+            acc += nu[i] * aii * jacobi_vals(c0, x, y, z, t, 0.0f);
+        }
+    }
+    const double s = block_reduce(acc, false, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// dL/dT per voxel from the adjoint row passes' accumulators (one thread per
+// (x, y) column, walking the heat slabs from the last):
+//   dL/ddk[h] = gA[h] + cnt(h) sum_{h' > h} gC[h']
+//   mu[h] = gnp[h] n_p'(T) + gns[h] n_s'(T) + dL/ddk[h] dk'(T)  (into gnp),
+// n' and dk' by central differences of the FP64 closed forms (step 1e-3 K);
+// plus `tc` at the voxel whose temperature sets the pump, the propagators and
+// the power normalisation (slab 0, grid centre).
+__global__ __launch_bounds__(256) void k_adj_heat_grad(const double* __restrict__ T, double* __restrict__ gnp,
+                                                       const double* __restrict__ gns, const double* __restrict__ gA,
+                                                       const double* __restrict__ gC, int nx, int ny, int nzh, int nz,
+                                                       int crystal, double lp, double Lambda, double ea, double eb,
+                                                       double tc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n2 = nx * ny;
+    if (i >= n2) return;
+    const double ls = 0.5 * lp, hT = 1e-3;
+    double suffix = 0.0;
+    for (int h = nzh - 1; h >= 0; h--) {
+        const size_t v = (size_t)h * n2 + i;
+        const double t = T[v];
+        const double npd = (refr_index<double>(crystal, lp, t + hT) - refr_index<double>(crystal, lp, t - hT)) / (2 * hT);
+        const double nsd = (refr_index<double>(crystal, ls, t + hT) - refr_index<double>(crystal, ls, t - hT)) / (2 * hT);
+        const double dkd = (delta_k(crystal, lp, Lambda, ea, eb, t + hT) - delta_k(crystal, lp, Lambda, ea, eb, t - hT)) /
+                           (2 * hT);
+        const long long s0 = ((long long)h * nz + nzh - 1) / nzh;
+        const long long s1 = ((long long)(h + 1) * nz + nzh - 1) / nzh;
+        const double ddk = gA[v] + (double)(s1 - s0) * suffix;
+        suffix += gC[v];
+        double m = gnp[v] * npd + gns[v] * nsd + ddk * dkd;
+        if (h == 0 && i == (ny / 2) * nx + nx / 2) m += tc;
+        gnp[v] = m;
+    }
+}
+
+// ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
 // x-y plane tiles of 64 x 4 columns, and z chunks of at least 48 slices (at
@@ -443,6 +545,27 @@ hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* n
                           int nz, int crystal, double lp, double Lambda, double ea, double eb) {
     hipLaunchKernelGGL(k_heat_dk, dim3((n2 + 255) / 256), dim3(256), 0, st, T, dk, nn, base, n2, nzh, nz, crystal, lp,
                        Lambda, ea, eb);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_heat_in(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu, double* u) {
+    const dim3 pg = plane_grid(c);
+    hipLaunchKernelGGL(k_adj_heat_in, dim3((c.nxu + 63) / 64, (c.nyu + 3) / 4, pg.z), dim3(64, 4), 0, st, c, a, mu, u);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_heat_out(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu,
+                               const double* nu, double* w, double* part) {
+    hipLaunchKernelGGL(k_adj_heat_out, plane_grid(c), dim3(64, 4), 0, st, c, a, mu, nu, w, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_heat_grad(hipStream_t st, const double* T, double* gnp, const double* gns, const double* gA,
+                                const double* gC, int nx, int ny, int nzh, int nz, int crystal, double lp,
+                                double Lambda, double ea, double eb, double tc) {
+    const int n2 = nx * ny;
+    hipLaunchKernelGGL(k_adj_heat_grad, dim3((n2 + 255) / 256), dim3(256), 0, st, T, gnp, gns, gA, gC, nx, ny, nzh, nz,
+                       crystal, lp, Lambda, ea, eb, tc);
     return hipGetLastError();
 }
 

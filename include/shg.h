@@ -250,13 +250,22 @@ SHG_API int shg_run_traced(shg_handle* h, const shg_params* p, int32_t every, sh
  * bit-identical to its shg_run result either way. */
 SHG_API int shg_run_batch(shg_handle* h, const shg_params* p, int32_t n, shg_result* r);
 
-/* Gradient of eta (non-THERMAL runs) by a hand-written adjoint of the
- * discrete SSFM/RK4 march with checkpointed recomputation.
+/* Gradient of eta by a hand-written adjoint of the discrete SSFM/RK4
+ * march with checkpointed recomputation.
  *   d_eta[0] = d eta / d waist_um
  *   d_eta[1] = d eta / d focal_um
- *   d_eta[2] = d eta / d T_C          (crystal temperature)
+ *   d_eta[2] = d eta / d T, with T = T_C = T_peltier_C moved together (the
+ *              reference's argv[3], src/cuSHG.cu:46-53): the crystal
+ *              temperature without THERMAL, the oven temperature with it
  *   d_eta[3 + s] = d eta / d poling_phase[s], s = 0..nz-1
- * d_eta must hold 3 + nz doubles.  checkpoint_every <= 0 selects a default.
+ * d_eta must hold 3 + nz doubles.  checkpoint_every <= 0 selects a default
+ * (with THERMAL: the whole last march when its history fits in a quarter of
+ * the free device memory).
+ * THERMAL runs (SHG_HEAT_DIRECT only) are differentiated at the converged
+ * thermo-optic steady state of the reference's outer loop
+ * (src/headers/Solver.h:321-373): the implicit-function adjoint through the
+ * heat solve, the heat source and the temperature dependence of n and dk,
+ * solved by Anderson-accelerated fixed-point sweeps.
  * Arithmetic: FP64 for the FP64 forms; every FP32-field form is
  * differentiated with FP32 fields and FP64 arithmetic (the FP32_STORAGE
  * march), so they give the same result.
