@@ -30,6 +30,8 @@ The JSON line also carries:
   other_fused_form     the other FP32-field form's value and eta error.
   adjoint         forward + adjoint (shg_adjoint) wall time over one forward
                   at 256x256 x 2000 slices.
+  adjoint_thermal the same for the THERMAL 512x512x4000 case, and one
+                  eta-maximisation step over 16 (waist, focus, oven) triples.
   cpu_baseline    the FP64 oracle (OpenMP, all host threads) on a bounded
                   sample of the first instance.
 """
@@ -93,6 +95,49 @@ def adjoint_cost(h) -> dict:
 # value form per workload when --precision is not given: the fastest form
 # whose eta is within 1e-5 of the FP64 plain form on every instance
 DEFAULT_FORM = {"sweep64": "storage"}
+
+
+def adjoint_thermal_cost(h, batch: int) -> dict:
+    """THERMAL gradient cost at the full coupled case: one MgO:PPLN
+    512x512x4000 instance with a 512x512x400 heat grid (configs.ppln512_thermal),
+    FP32 fields / FP64 arithmetic -- forward, then forward + adjoint
+    (shg_adjoint at the converged steady state), wall times of the second of
+    two calls; then, with batch > 0, one eta-maximisation step
+    (cushg_amd.optimize.ascent_step) over that many seeded (waist, focus,
+    oven) triples of the same case."""
+    from cushg_amd import abi, configs, optimize
+    p = configs.ppln512_thermal()
+    p.precision = abi.PREC_FP32_STORAGE
+    h.run(p, fields=False)
+    t = time.perf_counter()
+    rf, _ = h.run(p, fields=False)
+    tf = time.perf_counter() - t
+    h.adjoint(p)
+    t = time.perf_counter()
+    h.adjoint(p)
+    ta = time.perf_counter() - t
+    print(f"bench: THERMAL adjoint {ta * 1e3:.1f} ms vs forward {tf * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    out = {"workload": "512x512, 4000 slices, 50 mm MgO:PPLN, 30 W, THERMAL, heat 512x512x400, FP32 fields / FP64 "
+                       "arithmetic",
+           "forward_ms": round(tf * 1e3, 1), "forward_outer_passes": rf.outer_iters,
+           "forward_plus_adjoint_ms": round(ta * 1e3, 1), "ratio": round(ta / tf, 3),
+           "adjoint_only_over_forward": round((ta - tf) / tf, 3), "target": 3.0}
+    if batch > 0:
+        ps = configs.ppln512_thermal_batch(batch)
+        for q in ps:
+            q.precision = abi.PREC_FP32_STORAGE
+        st = optimize.ascent_step(h, ps)
+        rc, rb, _ = h.run_batch(ps)
+        t = time.perf_counter()
+        rc, rb, _ = h.run_batch(ps)
+        tb = time.perf_counter() - t
+        gains = [s.eta1 / s.eta0 - 1 for s in st.steps]
+        out["ascent_step"] = {"instances": batch, "forward_plus_adjoint_s": round(st.grad_wall_s, 2),
+                              "step_s": round(st.wall_s, 2), "batched_forward_s": round(tb, 2),
+                              "accepted": sum(s.accepted for s in st.steps),
+                              "eta_gain_rel": {"min": min(gains), "max": max(gains)}}
+        print(f"bench: ascent step over {batch}: {st.wall_s:.1f} s", file=sys.stderr, flush=True)
+    return out
 
 
 def workload(name: str):
@@ -268,7 +313,9 @@ def main():
                     help="before timing, run shg_autotune for every grid size of the workload (batch = the "
                          "workload's instances of that size); the tuning time is not in the timed region")
     ap.add_argument("--no-adjoint", action="store_true",
-                    help="skip the adjoint cost line (256x256, 2000 slices, FP32 storage)")
+                    help="skip the adjoint cost lines (256x256, 2000 slices; and THERMAL 512x512x4000)")
+    ap.add_argument("--ascent-batch", type=int, default=16,
+                    help="instances of the THERMAL eta-maximisation step timed in the adjoint line (0: none)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -383,6 +430,7 @@ def main():
         extra["other_fused_form"] = alt
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_cost(h)
+        extra["adjoint_thermal"] = adjoint_thermal_cost(h, args.ascent_batch)
 
     line = {
         "metric": METRIC,
