@@ -58,7 +58,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp, *objs, f"--offload-arch={ARCH}", "-fPIC", "-lrocblas", "-lhipblaslt"]
+    cmd = [HIPCC, "-shared", "-o", tmp, *objs, f"--offload-arch={ARCH}", "-fPIC"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}")
@@ -102,7 +102,7 @@ def build_sanitized(verbose: bool = False, jobs: int | None = None) -> str:
     jobs = jobs or min(8, os.cpu_count() or 4, len(srcs))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile_san, srcs))
-    cmd = [HIPCC, "-shared", "-o", SAN_LIB, *objs, f"--offload-arch={ARCH}", "-fPIC", "-lrocblas", "-lhipblaslt"]
+    cmd = [HIPCC, "-shared", "-o", SAN_LIB, *objs, f"--offload-arch={ARCH}", "-fPIC"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link (sanitize) failed:\n{r.stdout}")

@@ -10,7 +10,6 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
-#include <rocblas/rocblas.h>
 
 namespace shg {
 const char* last_error();
@@ -248,7 +247,6 @@ int shg_handle_destroy(shg_handle* h) {
     if (h->arena.base) (void)hipFree(h->arena.base);
     if (h->d_scalars) (void)hipFree(h->d_scalars);
     if (h->d_partials) (void)hipFree(h->d_partials);
-    if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
     heat_cache_free(h);
     for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto s : h->lanes)

@@ -11,8 +11,6 @@
 //                     fast diagonalisation (three axis transforms = plain
 //                     GEMMs, FP32 in the mixed form, FP64 in the plain form),
 //                     repeated until max|update| < heat_tol_K.
-#include <hipblaslt/hipblaslt.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -190,11 +188,18 @@ static bool axis_build(int n, double h, bool robin_lo, bool robin_hi, bool diric
 // Device-side tables of the direct solver (carved from the run arena).
 struct DirectTables {
     HeatAxes ax{};
-    const void* q[3] = {nullptr, nullptr, nullptr};  // eigenvector matrices (R precision)
+    const double* q[3] = {nullptr, nullptr, nullptr};   // FP64 eigenvector matrices
+    const double* qt[3] = {nullptr, nullptr, nullptr};  // and their (per-block) transposes
     bool fold[3] = {false, false, false};           // axis transformed in its even/odd fold
     void* u = nullptr;
     void* v = nullptr;
     bool fp64 = false;
+    // FP32 transforms on the matrix cores: split eigenbases (per axis: hi
+    // and lo of the forward operator, then of the inverse, nb blocks of
+    // rpad x kpad halves each), their scale exponents, and the max|u| slots
+    const _Float16* qs[3] = {nullptr, nullptr, nullptr};
+    int qe[3] = {}, rpad[3] = {}, kpad[3] = {};
+    unsigned* amax = nullptr;
 };
 
 // Per-geometry tables of the direct solver (axis widths, stiffness
@@ -203,9 +208,11 @@ struct DirectTables {
 struct HeatGeom {
     HeatAxes ax{};
     bool fold[3] = {};
-    void* mem[9] = {};
+    void* mem[12] = {};
     double* q64[3] = {};
-    float* q32[3] = {};
+    double* q64t[3] = {};
+    _Float16* qs[3] = {};
+    int qe[3] = {}, rpad[3] = {}, kpad[3] = {};
 };
 
 static std::string geom_key(const shg_params* p, const HeatState& hs) {
@@ -257,20 +264,45 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
         *kp[i] = d;
         *wp[i] = d + n;
         *lp[i] = d + 2 * n;
+        const int nb = axes[i].fold ? 2 : 1, nblk = n / nb;
         double* q64 = nullptr;
-        float* q32 = nullptr;
+        double* q64t = nullptr;
         fail |= hipMalloc(reinterpret_cast<void**>(&q64), 8 * nq) != hipSuccess;
         if (fail) break;
         g->mem[3 * i + 1] = q64;
-        fail |= hipMalloc(reinterpret_cast<void**>(&q32), 4 * nq) != hipSuccess;
+        fail |= hipMalloc(reinterpret_cast<void**>(&q64t), 8 * nq) != hipSuccess;
         if (fail) break;
-        g->mem[3 * i + 2] = q32;
-        std::vector<float> qf(axes[i].q.begin(), axes[i].q.end());
+        g->mem[3 * i + 2] = q64t;
+        std::vector<double> qt(nq);
+        for (int bl = 0; bl < nb; bl++)
+            for (int r = 0; r < nblk; r++)
+                for (int cc = 0; cc < nblk; cc++)
+                    qt[(size_t)bl * nblk * nblk + (size_t)r * nblk + cc] =
+                        axes[i].q[(size_t)bl * nblk * nblk + (size_t)cc * nblk + r];
         fail |= hipMemcpy(q64, axes[i].q.data(), 8 * nq, hipMemcpyHostToDevice) != hipSuccess;
-        fail |= hipMemcpy(q32, qf.data(), 4 * nq, hipMemcpyHostToDevice) != hipSuccess;
+        fail |= hipMemcpy(q64t, qt.data(), 8 * nq, hipMemcpyHostToDevice) != hipSuccess;
         g->q64[i] = q64;
-        g->q32[i] = q32;
+        g->q64t[i] = q64t;
+        // the split operators of the FP32 (matrix-core FP16) transforms
+        const int rpad = (nblk + 127) / 128 * 128, kpad = (nblk + 31) / 32 * 32;
+        double qm = 0;
+        for (double v : axes[i].q) qm = std::max(qm, std::fabs(v));
+        int e = 0;
+        (void)std::frexp(qm > 0 ? qm : 1.0, &e);
+        const size_t blk = (size_t)nb * rpad * kpad;
+        _Float16* qs = nullptr;
+        fail |= hipMalloc(reinterpret_cast<void**>(&qs), 2 * 4 * blk) != hipSuccess;
+        if (fail) break;
+        g->mem[9 + i] = qs;
+        g->qs[i] = qs;
+        g->qe[i] = 14 - e;
+        g->rpad[i] = rpad;
+        g->kpad[i] = kpad;
+        fail |= launch_split_basis(nullptr, q64, nblk, nb, false, g->qe[i], qs, qs + blk, rpad, kpad) != hipSuccess;
+        fail |= launch_split_basis(nullptr, q64, nblk, nb, true, g->qe[i], qs + 2 * blk, qs + 3 * blk, rpad, kpad) !=
+                hipSuccess;
     }
+    if (!fail) fail |= hipDeviceSynchronize() != hipSuccess;
     if (fail) {
         (void)hipGetLastError();  // not left for a later launch check to report
         for (void* m : g->mem)
@@ -284,14 +316,7 @@ static HeatGeom* heat_geom(shg_handle* h, const shg_params* p, const HeatState& 
     return g;
 }
 
-struct LtPlan;
-static void lt_plan_free(LtPlan* p);
-
 void heat_cache_free(shg_handle* h) {
-    for (auto& kv : h->lt_plans) lt_plan_free(static_cast<LtPlan*>(kv.second));
-    h->lt_plans.clear();
-    if (h->blaslt) hipblasLtDestroy((hipblasLtHandle_t)h->blaslt);
-    h->blaslt = nullptr;
     for (auto& kv : h->heat_cache) {
         for (void* m : kv.second->mem)
             if (m) (void)hipFree(m);
@@ -311,20 +336,10 @@ size_t heat_bytes(const shg_params* p) {
     size_t b = 4 * al256(8 * n3) + al256(8 * n3) + al256(4 * n3);  // T, Tn, dk, dkbase; nn, Q
     b += 2 * al256(rs * nu);                           // u, v
     const size_t nb = (size_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * nzh;
-    b += al256(8 * nb) + al256(8 * 8);                 // partials, slots
+    b += al256(8 * nb) + al256(8 * 8) + al256(4 * 8);  // partials, slots, max|u| slots
     b += al256(8 * (size_t)p->nz);                     // poling
     b += 4096;
     return b;
-}
-
-static rocblas_handle blas(shg_handle* h) {
-    if (!h->blas) {
-        rocblas_handle bh = nullptr;
-        if (rocblas_create_handle(&bh) != rocblas_status_success) return nullptr;
-        rocblas_set_stream(bh, h->stream);
-        h->blas = bh;
-    }
-    return (rocblas_handle)h->blas;
 }
 
 int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
@@ -359,7 +374,8 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     dt->fp64 = f64;
     dt->u = f64 ? (void*)A.carve<double>(nu) : (void*)A.carve<float>(nu);
     dt->v = f64 ? (void*)A.carve<double>(nu) : (void*)A.carve<float>(nu);
-    if (!hs.T || !hs.Q || !hs.red || !dt->u || !dt->v) {
+    dt->amax = A.carve<unsigned>(8);
+    if (!hs.T || !hs.Q || !hs.red || !dt->u || !dt->v || !dt->amax) {
         set_error("arena exhausted (heat buffers)");
         return SHG_ENOMEM;
     }
@@ -369,13 +385,12 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     dt->ax = g->ax;
     for (int i = 0; i < 3; i++) {
         dt->fold[i] = g->fold[i];
-        if (f64) {
-            if (!g->q64[i]) return SHG_ENOMEM;
-            dt->q[i] = g->q64[i];
-        } else {
-            if (!g->q32[i]) return SHG_ENOMEM;
-            dt->q[i] = g->q32[i];
-        }
+        dt->qs[i] = g->qs[i];
+        dt->qe[i] = g->qe[i];
+        dt->rpad[i] = g->rpad[i];
+        dt->kpad[i] = g->kpad[i];
+        dt->q[i] = g->q64[i];
+        dt->qt[i] = g->q64t[i];
     }
     return SHG_OK;
 }
@@ -400,138 +415,57 @@ static int read_scalar(shg_handle* h, const double* d, double* out) {
 
 // out[..i..] = sum_j M[j][i] in[..j..] along one axis of u[nz][nyu][nx]
 // (M = Q forward, Q^T inverse; Q row-major, Q[j*n+i] = component j of vector
-// i).  On a folded axis (x or z) the data are in the even/odd fold and Q is
-// the even block followed by the odd one: two products of half the length,
-// one batched call.
-template <typename R>
-static rocblas_status gemm_sb(rocblas_handle bh, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
-                              const R* a, int lda, rocblas_stride sa, const R* b, int ldb, rocblas_stride sb, R* c,
-                              int ldc, rocblas_stride sc, int batch) {
-    const R one = 1, zero = 0;
-    if (batch == 1) {
-        if constexpr (sizeof(R) == 4)
-            return rocblas_sgemm(bh, ta, tb, m, n, k, &one, a, lda, b, ldb, &zero, c, ldc);
-        else
-            return rocblas_dgemm(bh, ta, tb, m, n, k, &one, a, lda, b, ldb, &zero, c, ldc);
-    }
-    if constexpr (sizeof(R) == 4)
-        return rocblas_sgemm_strided_batched(bh, ta, tb, m, n, k, &one, a, lda, sa, b, ldb, sb, &zero, c, ldc, sc,
-                                             batch);
-    else
-        return rocblas_dgemm_strided_batched(bh, ta, tb, m, n, k, &one, a, lda, sa, b, ldb, sb, &zero, c, ldc, sc,
-                                             batch);
-}
-
-// The z transform in FP32 goes through hipBLASLt: for its shape (M = nx nyu,
-// N = K = nz or nz/2) the algorithm hipBLASLt's heuristic ranks first runs
-// about twice as fast as rocBLAS's choice (365 against 714 us folded at
-// 512 x 511 x 400, measured on MI355X); rocBLAS stays faster for the x and y
-// transforms.  One descriptor set and algorithm per shape, cached on the
-// handle; no workspace.  A shape hipBLASLt has no algorithm for stays on
-// rocBLAS.
-struct LtPlan {
-    bool ok = false;
-    hipblasLtMatmulDesc_t d = nullptr;
-    hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
-    hipblasLtMatmulAlgo_t algo{};
-};
-static void lt_plan_free(LtPlan* p) {
-    if (!p) return;
-    if (p->la) hipblasLtMatrixLayoutDestroy(p->la);
-    if (p->lb) hipblasLtMatrixLayoutDestroy(p->lb);
-    if (p->lc) hipblasLtMatrixLayoutDestroy(p->lc);
-    if (p->d) hipblasLtMatmulDescDestroy(p->d);
-    delete p;
-}
-// C[M x N] = A[M x K] op(B), column-major, `batch` strided products
-static LtPlan* lt_plan(shg_handle* h, bool tb, int M, int N, int K, long sa, long sb, long sc, int batch) {
-    char key[160];
-    snprintf(key, sizeof key, "%d/%d/%d/%d/%ld/%ld/%ld/%d", (int)tb, M, N, K, sa, sb, sc, batch);
-    auto it = h->lt_plans.find(key);
-    if (it != h->lt_plans.end()) return static_cast<LtPlan*>(it->second);
-    LtPlan* p = new LtPlan();
-    h->lt_plans[key] = p;
-    if (!h->blaslt) {
-        hipblasLtHandle_t lt = nullptr;
-        if (hipblasLtCreate(&lt) != HIPBLAS_STATUS_SUCCESS) return p;
-        h->blaslt = lt;
-    }
-    const hipblasOperation_t opn = HIPBLAS_OP_N, opb = tb ? HIPBLAS_OP_T : HIPBLAS_OP_N;
-    bool ok = hipblasLtMatmulDescCreate(&p->d, HIPBLAS_COMPUTE_32F, HIP_R_32F) == HIPBLAS_STATUS_SUCCESS &&
-              hipblasLtMatmulDescSetAttribute(p->d, HIPBLASLT_MATMUL_DESC_TRANSA, &opn, sizeof opn) ==
-                  HIPBLAS_STATUS_SUCCESS &&
-              hipblasLtMatmulDescSetAttribute(p->d, HIPBLASLT_MATMUL_DESC_TRANSB, &opb, sizeof opb) ==
-                  HIPBLAS_STATUS_SUCCESS &&
-              hipblasLtMatrixLayoutCreate(&p->la, HIP_R_32F, M, K, M) == HIPBLAS_STATUS_SUCCESS &&
-              hipblasLtMatrixLayoutCreate(&p->lb, HIP_R_32F, tb ? N : K, tb ? K : N, tb ? N : K) ==
-                  HIPBLAS_STATUS_SUCCESS &&
-              hipblasLtMatrixLayoutCreate(&p->lc, HIP_R_32F, M, N, M) == HIPBLAS_STATUS_SUCCESS;
-    const int32_t bc = batch;
-    const int64_t so[3] = {sa, sb, sc};
-    hipblasLtMatrixLayout_t ls[3] = {p->la, p->lb, p->lc};
-    for (int i = 0; i < 3 && ok; i++)
-        ok = hipblasLtMatrixLayoutSetAttribute(ls[i], HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof bc) ==
-                 HIPBLAS_STATUS_SUCCESS &&
-             hipblasLtMatrixLayoutSetAttribute(ls[i], HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &so[i],
-                                               sizeof so[i]) == HIPBLAS_STATUS_SUCCESS;
-    if (ok) {
-        hipblasLtMatmulPreference_t pref = nullptr;
-        const uint64_t ws = 0;
-        hipblasLtMatmulHeuristicResult_t res{};
-        int n = 0;
-        ok = hipblasLtMatmulPreferenceCreate(&pref) == HIPBLAS_STATUS_SUCCESS &&
-             hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &ws, sizeof ws) ==
-                 HIPBLAS_STATUS_SUCCESS &&
-             hipblasLtMatmulAlgoGetHeuristic((hipblasLtHandle_t)h->blaslt, p->d, p->la, p->lb, p->lc, p->lc, pref, 1,
-                                             &res, &n) == HIPBLAS_STATUS_SUCCESS &&
-             n > 0;
-        if (pref) hipblasLtMatmulPreferenceDestroy(pref);
-        if (ok) p->algo = res.algo;
-    }
-    p->ok = ok;
-    return p;
-}
-
-template <typename R>
-static int axis_gemm(shg_handle* h, int axis, bool fwd, bool fold, const R* q, const R* in, R* out, int nx, int nyu,
-                     int nz) {
-    rocblas_handle bh = blas(h);
-    if (!bh) {
-        set_error("rocBLAS handle creation failed");
-        return SHG_EDEVICE;
-    }
-    rocblas_status st;
-    const rocblas_operation N_ = rocblas_operation_none, T_ = rocblas_operation_transpose;
-    const rocblas_stride plane = (rocblas_stride)nx * nyu;
+// i, qt its transpose).  On a folded axis (x or z) the data are in the
+// even/odd fold and Q is the even block followed by the odd one: two
+// products of half the length, one batched launch.  FP64, as R S products
+// of row-major operands (kernels_heat_gemm.hip):
+//   x: out[row][i] = sum_k in[row][k] Q_op[i][k]   (S = q forward, qt inverse)
+//   y, z: out[i][col] = sum_k Q_op[i][k] in[k][col] (R = qt forward, q inverse)
+static int axis_gemm64(shg_handle* h, int axis, bool fwd, bool fold, const double* q, const double* qt,
+                       const double* in, double* out, int nx, int nyu, int nz) {
+    const long plane = (long)nx * nyu;
+    HeatGemm64 g{};
+    int batch = 1;
     if (axis == 0) {
-        const int n = fold ? nx / 2 : nx;  // per block
-        st = gemm_sb<R>(bh, fwd ? N_ : T_, N_, n, nyu * nz, n, q, n, (rocblas_stride)n * n, in, nx, n, out, nx, n,
-                        fold ? 2 : 1);
+        const int n = fold ? nx / 2 : nx;
+        g.r = in;
+        g.s = fwd ? q : qt;
+        g.c = out;
+        g.ldr = g.ldc = nx;
+        g.lds = n;
+        g.sr = g.sc = n;
+        g.ss = (long)n * n;
+        g.I = (long)nyu * nz;
+        g.J = n;
+        g.n = n;
+        batch = fold ? 2 : 1;
     } else if (axis == 1) {
-        st = gemm_sb<R>(bh, N_, fwd ? T_ : N_, nx, nyu, nyu, in, nx, plane, q, nyu, 0, out, nx, plane, nz);
+        g.r = fwd ? qt : q;
+        g.s = in;
+        g.c = out;
+        g.ldr = nyu;
+        g.lds = g.ldc = nx;
+        g.sr = 0;
+        g.ss = g.sc = plane;
+        g.I = nyu;
+        g.J = nx;
+        g.n = nyu;
+        batch = nz;
     } else {
         const int n = fold ? nz / 2 : nz;
-        if constexpr (sizeof(R) == 4) {
-            LtPlan* lp = lt_plan(h, fwd, nx * nyu, n, n, plane * n, (long)n * n, plane * n, fold ? 2 : 1);
-            if (lp->ok) {
-                const float one = 1.0f, zero = 0.0f;
-                const hipblasStatus_t ls =
-                    hipblasLtMatmul((hipblasLtHandle_t)h->blaslt, lp->d, &one, in, lp->la, q, lp->lb, &zero, out,
-                                    lp->lc, out, lp->lc, &lp->algo, nullptr, 0, h->stream);
-                if (ls != HIPBLAS_STATUS_SUCCESS) {
-                    set_error("hipBLASLt matmul failed (%d) on axis 2", (int)ls);
-                    return SHG_EDEVICE;
-                }
-                return SHG_OK;
-            }
-        }
-        st = gemm_sb<R>(bh, N_, fwd ? T_ : N_, nx * nyu, n, n, in, nx * nyu, plane * n, q, n, (rocblas_stride)n * n,
-                        out, nx * nyu, plane * n, fold ? 2 : 1);
+        g.r = fwd ? qt : q;
+        g.s = in;
+        g.c = out;
+        g.ldr = n;
+        g.lds = g.ldc = plane;
+        g.sr = (long)n * n;
+        g.ss = g.sc = plane * n;
+        g.I = n;
+        g.J = plane;
+        g.n = n;
+        batch = fold ? 2 : 1;
     }
-    if (st != rocblas_status_success) {
-        set_error("rocBLAS gemm failed (%d) on axis %d", (int)st, axis);
-        return SHG_EDEVICE;
-    }
+    SHG_HIP(launch_heat_gemm64(h->stream, g, batch));
     return SHG_OK;
 }
 
@@ -547,6 +481,59 @@ static int resid_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTab
     return read_scalar(h, hs.red + 0, r_sq);
 }
 
+// One FP32 eigenbasis transform along `axis` of u[nz][nyu][nx] on the
+// matrix cores (split-FP16 products, kernels_heat_gemm.hip): the same
+// products as axis_gemm, with max|in| read from umax and max|out| written
+// to cmax (when given) for the next transform's scaling.
+static int axis_mfma(shg_handle* h, int axis, bool fwd, const DirectTables& dt, const float* in, float* out, int nx,
+                     int nyu, int nz, const unsigned* umax, unsigned* cmax) {
+    const bool fold = dt.fold[axis];
+    const int nb = fold ? 2 : 1;
+    const long blk = (long)dt.rpad[axis] * dt.kpad[axis];
+    const _Float16* base = dt.qs[axis] + (fwd ? 0 : 2) * nb * blk;
+    const long plane = (long)nx * nyu;
+    HeatGemm g{};
+    g.u = in;
+    g.c = out;
+    g.qh = base;
+    g.ql = base + nb * blk;
+    g.umax = umax;
+    g.cmax = cmax;
+    g.qpad = dt.kpad[axis];
+    g.qe = dt.qe[axis];
+    int batch = nb;
+    if (axis == 0) {  // C[row][m] = sum_k u[row][k] Q_op[m][k], rows = (z, y)
+        g.kindB = 1;
+        g.n = fold ? nx / 2 : nx;
+        g.nbig = (long)nyu * nz;
+        g.ldu = g.ldc = nx;
+        g.su = g.sc = g.n;
+        g.sq = blk;
+    } else if (axis == 1) {  // per z plane: C[m][x] = sum_k Q_op[m][k] u[k][x]
+        g.kindB = 0;
+        g.n = nyu;
+        g.nbig = nx;
+        g.ldu = g.ldc = nx;
+        g.su = g.sc = plane;
+        g.sq = 0;
+        batch = nz;
+    } else {  // C[m][(y, x)] = sum_k Q_op[m][k] u[k][(y, x)]
+        g.kindB = 0;
+        g.n = fold ? nz / 2 : nz;
+        g.nbig = plane;
+        g.ldu = g.ldc = plane;
+        g.su = g.sc = plane * g.n;
+        g.sq = blk;
+    }
+    if (g.n > dt.rpad[axis] || g.n > dt.kpad[axis]) {
+        set_error("heat: transform length %d exceeds its split eigenbasis (%d x %d)", g.n, dt.rpad[axis],
+                  dt.kpad[axis]);
+        return SHG_EDEVICE;
+    }
+    SHG_HIP(launch_heat_gemm(h->stream, g, batch));
+    return SHG_OK;
+}
+
 // correction pass: T += W^-1/2 S^-1 u (u from the residual pass); returns max|update|
 template <typename R>
 static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* max_corr) {
@@ -554,22 +541,38 @@ static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectT
     const bool fx = dt.fold[0], fz = dt.fold[2];
     R* a = static_cast<R*>(dt.u);
     R* b = static_cast<R*>(dt.v);
-    const R* qx = static_cast<const R*>(dt.q[0]);
-    const R* qy = static_cast<const R*>(dt.q[1]);
-    const R* qz = static_cast<const R*>(dt.q[2]);
-    {
+    if constexpr (sizeof(R) == 4) {
+        // slots: 0 the folded input, 1-2 after the x and y transforms,
+        // 3 after the division, 4-5 after the inverse z and y transforms
+        unsigned* m = dt.amax;
+        ProfScope ps(h, "heat_fd_gemm");
+        SHG_HIP(launch_zero_u32(h->stream, m, 8));
+        if (fx || fz) {
+            SHG_HIP(launch_fd_fold<R>(h->stream, a, b, nx, nyu, nz, fx, fz, m + 0));
+            std::swap(a, b);
+        } else {
+            SHG_HIP(launch_absmax(h->stream, a, (long)nx * nyu * nz, m + 0));
+        }
+        if (int rc = axis_mfma(h, 0, true, dt, a, b, nx, nyu, nz, m + 0, m + 1)) return rc;
+        if (int rc = axis_mfma(h, 1, true, dt, b, a, nx, nyu, nz, m + 1, m + 2)) return rc;
+        if (int rc = axis_mfma(h, 2, true, dt, a, b, nx, nyu, nz, m + 2, nullptr)) return rc;
+        SHG_HIP(launch_fd_divide<R>(h->stream, b, dt.ax, nx, nyu, nz, m + 3));
+        if (int rc = axis_mfma(h, 2, false, dt, b, a, nx, nyu, nz, m + 3, m + 4)) return rc;
+        if (int rc = axis_mfma(h, 1, false, dt, a, b, nx, nyu, nz, m + 4, m + 5)) return rc;
+        if (int rc = axis_mfma(h, 0, false, dt, b, a, nx, nyu, nz, m + 5, nullptr)) return rc;
+    } else {
         ProfScope ps(h, "heat_fd_gemm");
         if (fx || fz) {
             SHG_HIP(launch_fd_fold<R>(h->stream, a, b, nx, nyu, nz, fx, fz));
             std::swap(a, b);
         }
-        if (int rc = axis_gemm<R>(h, 0, true, fx, qx, a, b, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 1, true, false, qy, b, a, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 2, true, fz, qz, a, b, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm64(h, 0, true, fx, dt.q[0], dt.qt[0], a, b, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm64(h, 1, true, false, dt.q[1], dt.qt[1], b, a, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm64(h, 2, true, fz, dt.q[2], dt.qt[2], a, b, nx, nyu, nz)) return rc;
         SHG_HIP(launch_fd_divide<R>(h->stream, b, dt.ax, nx, nyu, nz));
-        if (int rc = axis_gemm<R>(h, 2, false, fz, qz, b, a, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 1, false, false, qy, a, b, nx, nyu, nz)) return rc;
-        if (int rc = axis_gemm<R>(h, 0, false, fx, qx, b, a, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm64(h, 2, false, fz, dt.q[2], dt.qt[2], b, a, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm64(h, 1, false, false, dt.q[1], dt.qt[1], a, b, nx, nyu, nz)) return rc;
+        if (int rc = axis_gemm64(h, 0, false, fx, dt.q[0], dt.qt[0], b, a, nx, nyu, nz)) return rc;
     }
     {
         ProfScope ps(h, "heat_update");
@@ -678,7 +681,7 @@ int heat_adjoint_solve(shg_handle* h, const shg_params* p, HeatState& hs, const 
     int grc = SHG_OK;
     HeatGeom* g = heat_geom(h, p, hs, &grc);
     if (!g) return grc;
-    if (!g->q64[0] || !g->q64[1] || !g->q64[2]) {
+    if (!g->q64[0] || !g->q64[1] || !g->q64[2] || !g->q64t[0] || !g->q64t[1] || !g->q64t[2]) {
         set_error("heat adjoint: FP64 eigenbases missing");
         return SHG_ENOMEM;
     }
@@ -694,13 +697,13 @@ int heat_adjoint_solve(shg_handle* h, const shg_params* p, HeatState& hs, const 
     }
     // nu = W^-1/2 Q S^-1 Q^T W^-1/2 mu: the exact inverse, in FP64 (the
     // operator is symmetric, so the transposed solve is this one)
-    if (int rc = axis_gemm<double>(h, 0, true, fx, g->q64[0], a, b, nx, nyu, nz)) return rc;
-    if (int rc = axis_gemm<double>(h, 1, true, false, g->q64[1], b, a, nx, nyu, nz)) return rc;
-    if (int rc = axis_gemm<double>(h, 2, true, fz, g->q64[2], a, b, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm64(h, 0, true, fx, g->q64[0], g->q64t[0], a, b, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm64(h, 1, true, false, g->q64[1], g->q64t[1], b, a, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm64(h, 2, true, fz, g->q64[2], g->q64t[2], a, b, nx, nyu, nz)) return rc;
     SHG_HIP(launch_fd_divide<double>(h->stream, b, g->ax, nx, nyu, nz));
-    if (int rc = axis_gemm<double>(h, 2, false, fz, g->q64[2], b, a, nx, nyu, nz)) return rc;
-    if (int rc = axis_gemm<double>(h, 1, false, false, g->q64[1], a, b, nx, nyu, nz)) return rc;
-    if (int rc = axis_gemm<double>(h, 0, false, fx, g->q64[0], b, a, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm64(h, 2, false, fz, g->q64[2], g->q64t[2], b, a, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm64(h, 1, false, false, g->q64[1], g->q64t[1], a, b, nx, nyu, nz)) return rc;
+    if (int rc = axis_gemm64(h, 0, false, fx, g->q64[0], g->q64t[0], b, a, nx, nyu, nz)) return rc;
     SHG_HIP(hipMemsetAsync(nu, 0, sizeof(double) * (size_t)c.nx * c.ny * nz, h->stream));
     int nb = 0;
     SHG_HIP(launch_fd_update<double>(h->stream, c, g->ax, a, nu, hs.part, fx, fz, &nb));

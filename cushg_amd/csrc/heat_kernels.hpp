@@ -32,6 +32,46 @@ struct HeatAxes {
     const double *lx, *ly, *lz;
 };
 
+// One eigenbasis transform on the matrix cores (kernels_heat_gemm.hip):
+// C[i][j] = sum_k R(i, k) S(k, j) over k < n, C row-major with row stride
+// ldc, `batch` independent blocks (strides su, sc of the data, sq of the
+// split eigenbasis);
+//   kind A (kindB = 0): R = Q_op [i][k] (i < n), S(k, j) = u[k ldu + j] (j < nbig)
+//   kind B (kindB = 1): R(i, k) = u[i ldu + k] (i < nbig), S(k, j) = Q_op [j][k] (j < n)
+// qh, ql: the eigenbasis in the direction asked for, split into FP16 hi and
+// lo parts of 2^qe q, [row][k] with rows padded to a multiple of 128 and k
+// (row stride qpad) to a multiple of 32; umax: max|u| as float bits; cmax
+// (optional): receives max|C| (atomic max on float bits, cleared before).
+struct HeatGemm {
+    const float* u;
+    float* c;
+    const _Float16* qh;
+    const _Float16* ql;
+    const unsigned* umax;
+    unsigned* cmax;
+    long ldu, ldc, su, sc, sq, nbig;
+    int n, qpad, qe, kindB;
+};
+hipError_t launch_heat_gemm(hipStream_t st, const HeatGemm& g, int batch);
+// the split operand of launch_heat_gemm from the FP64 eigenvectors q (nb
+// blocks of n x n, component j of vector i at q[j n + i]): Q_op[m][k] =
+// q[k n + m] forward, q[m n + k] inverse, times 2^qe; nb blocks of rpad x
+// kpad halves each in hi and lo.
+hipError_t launch_split_basis(hipStream_t st, const double* q, int n, int nb, bool inv, int qe, _Float16* hi,
+                              _Float16* lo, int rpad, int kpad);
+// FP64 transform C[i][j] = sum_k r[i ldr + k] s[k lds + j] (i < I, j < J,
+// k < n), `batch` blocks with strides sr, ss, sc.
+struct HeatGemm64 {
+    const double* r;
+    const double* s;
+    double* c;
+    long ldr, lds, ldc, sr, ss, sc, I, J;
+    int n;
+};
+hipError_t launch_heat_gemm64(hipStream_t st, const HeatGemm64& g, int batch);
+hipError_t launch_absmax(hipStream_t st, const float* a, long n, unsigned* out);
+hipError_t launch_zero_u32(hipStream_t st, unsigned* p, int n);
+
 HeatCoef heat_coef(const shg_params* p, int nzh);
 int heat_grid_blocks(const HeatCoef& c);
 
@@ -40,11 +80,15 @@ hipError_t launch_heat_sweep(hipStream_t st, const HeatCoef& c, const double* Ti
 template <typename R>
 hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* T, const float* Q,
                              R* u, double* part_sq);
+// amax (optional, FP32 only): atomic max of |v| after the division, as float bits
 template <typename R>
-hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz);
-// even/odd fold of the x and/or z axis of u (in -> out; not in place)
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz,
+                            unsigned* amax = nullptr);
+// even/odd fold of the x and/or z axis of u (in -> out; not in place);
+// amax as for launch_fd_divide, of the output
 template <typename R>
-hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz);
+hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz,
+                          unsigned* amax = nullptr);
 // T += W^-1/2 u, u folded along x / z as launch_fd_fold leaves it; writes
 // *nblocks partial maxima (at most heat_grid_blocks(c))
 template <typename R>

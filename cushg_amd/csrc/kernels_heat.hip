@@ -175,21 +175,37 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
 // are multiplied by the FP32 reciprocal of the FP64 sum (within an ulp or two
 // of the division, the precision of the FP32 transforms around it); FP64
 // data are divided.
+// workgroup max of m (256 threads, every thread calling) into *amax
+__device__ __forceinline__ void block_amax(float m, unsigned* amax) {
+    __shared__ float wm[4];
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((t & 63) == 0) wm[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) atomicMax(amax, __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx,
                                                    const double* __restrict__ ly, const double* __restrict__ lz,
-                                                   int nxu, int nyu, int nz) {
+                                                   int nxu, int nyu, int nz, unsigned* __restrict__ amax) {
     const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
-    if (x >= nxu) return;
-    const int per = (nz + gridDim.z - 1) / gridDim.z, z0 = blockIdx.z * per, z1 = min(nz, z0 + per);
-    const double lxy = lx[x] + ly[j];
-    for (int z = z0; z < z1; z++) {
-        const size_t i = ((size_t)z * nyu + j) * nxu + x;
-        if constexpr (sizeof(R) == 4)
-            v[i] = v[i] * __builtin_amdgcn_rcpf((float)(lxy + lz[z]));
-        else
-            v[i] = v[i] / (lxy + lz[z]);
+    float m = 0.0f;
+    if (x < nxu) {
+        const int per = (nz + gridDim.z - 1) / gridDim.z, z0 = blockIdx.z * per, z1 = min(nz, z0 + per);
+        const double lxy = lx[x] + ly[j];
+        for (int z = z0; z < z1; z++) {
+            const size_t i = ((size_t)z * nyu + j) * nxu + x;
+            if constexpr (sizeof(R) == 4) {
+                const float r = v[i] * __builtin_amdgcn_rcpf((float)(lxy + lz[z]));
+                v[i] = r;
+                m = fmaxf(m, fabsf(r));
+            } else {
+                v[i] = v[i] / (lxy + lz[z]);
+            }
+        }
     }
+    if (amax) block_amax(m, amax);
 }
 
 // Even/odd fold of the symmetric axes of u[nz][nyu][nxu]: along a folded
@@ -203,30 +219,39 @@ constexpr double kRsqrt2 = 0.70710678118654752440;
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_fold(const R* __restrict__ in, R* __restrict__ out, int nxu, int nyu,
-                                                 int nz, int fx, int fz) {
+                                                 int nz, int fx, int fz, unsigned* __restrict__ amax) {
     const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
     const int xu = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
-    if (xu >= hx || j >= nyu || z >= hz) return;
-    const int xm = nxu - 1 - xu, zm = nz - 1 - z;
-    auto at = [&](int zi, int xi) -> size_t { return ((size_t)zi * nyu + j) * nxu + xi; };
-    if (fx && fz) {
-        const double a = in[at(z, xu)], b = in[at(z, xm)], c = in[at(zm, xu)], d = in[at(zm, xm)];
-        const double ez = (a + b) * kRsqrt2, oz = (a - b) * kRsqrt2, ezm = (c + d) * kRsqrt2, ozm = (c - d) * kRsqrt2;
-        out[at(z, xu)] = (R)((ez + ezm) * kRsqrt2);
-        out[at(hz + z, xu)] = (R)((ez - ezm) * kRsqrt2);
-        out[at(z, hx + xu)] = (R)((oz + ozm) * kRsqrt2);
-        out[at(hz + z, hx + xu)] = (R)((oz - ozm) * kRsqrt2);
-    } else if (fx) {
-        const double a = in[at(z, xu)], b = in[at(z, xm)];
-        out[at(z, xu)] = (R)((a + b) * kRsqrt2);
-        out[at(z, hx + xu)] = (R)((a - b) * kRsqrt2);
-    } else if (fz) {
-        const double a = in[at(z, xu)], c = in[at(zm, xu)];
-        out[at(z, xu)] = (R)((a + c) * kRsqrt2);
-        out[at(hz + z, xu)] = (R)((a - c) * kRsqrt2);
-    } else {
-        out[at(z, xu)] = in[at(z, xu)];
+    float m = 0.0f;
+    if (xu < hx && j < nyu && z < hz) {
+        const int xm = nxu - 1 - xu, zm = nz - 1 - z;
+        auto at = [&](int zi, int xi) -> size_t { return ((size_t)zi * nyu + j) * nxu + xi; };
+        auto put = [&](size_t i, double v) {
+            const R r = (R)v;
+            out[i] = r;
+            m = fmaxf(m, fabsf((float)r));
+        };
+        if (fx && fz) {
+            const double a = in[at(z, xu)], b = in[at(z, xm)], c = in[at(zm, xu)], d = in[at(zm, xm)];
+            const double ez = (a + b) * kRsqrt2, oz = (a - b) * kRsqrt2, ezm = (c + d) * kRsqrt2,
+                         ozm = (c - d) * kRsqrt2;
+            put(at(z, xu), (ez + ezm) * kRsqrt2);
+            put(at(hz + z, xu), (ez - ezm) * kRsqrt2);
+            put(at(z, hx + xu), (oz + ozm) * kRsqrt2);
+            put(at(hz + z, hx + xu), (oz - ozm) * kRsqrt2);
+        } else if (fx) {
+            const double a = in[at(z, xu)], b = in[at(z, xm)];
+            put(at(z, xu), (a + b) * kRsqrt2);
+            put(at(z, hx + xu), (a - b) * kRsqrt2);
+        } else if (fz) {
+            const double a = in[at(z, xu)], c = in[at(zm, xu)];
+            put(at(z, xu), (a + c) * kRsqrt2);
+            put(at(hz + z, xu), (a - c) * kRsqrt2);
+        } else {
+            put(at(z, xu), in[at(z, xu)]);
+        }
     }
+    if (amax) block_amax(m, amax);
 }
 
 // T += W^-1/2 u (the correction), u unfolded first along the folded axes
@@ -492,24 +517,26 @@ template hipError_t launch_heat_resid<double>(hipStream_t, const HeatCoef&, cons
                                               const float*, double*, double*);
 
 template <typename R>
-hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz) {
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz, unsigned* amax) {
     const int zc = std::max(1, std::min(nz, 16));
     hipLaunchKernelGGL((k_fd_divide<R>), dim3((nxu + 255) / 256, nyu, zc), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu,
-                       nyu, nz);
+                       nyu, nz, amax);
     return hipGetLastError();
 }
-template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int);
-template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxes&, int, int, int);
+template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int, unsigned*);
+template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxes&, int, int, int, unsigned*);
 
 template <typename R>
-hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz) {
+hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz,
+                          unsigned* amax) {
     const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
     hipLaunchKernelGGL((k_fd_fold<R>), dim3((hx + 63) / 64, (nyu + 3) / 4, hz), dim3(64, 4), 0, st, in, out, nxu, nyu,
-                       nz, fx ? 1 : 0, fz ? 1 : 0);
+                       nz, fx ? 1 : 0, fz ? 1 : 0, amax);
     return hipGetLastError();
 }
-template hipError_t launch_fd_fold<float>(hipStream_t, const float*, float*, int, int, int, bool, bool);
-template hipError_t launch_fd_fold<double>(hipStream_t, const double*, double*, int, int, int, bool, bool);
+template hipError_t launch_fd_fold<float>(hipStream_t, const float*, float*, int, int, int, bool, bool, unsigned*);
+template hipError_t launch_fd_fold<double>(hipStream_t, const double*, double*, int, int, int, bool, bool,
+                                           unsigned*);
 
 // The grid keeps the block count of plane_grid (the size of the partials
 // array) at most: the z chunks take the x blocks a fold saves.

@@ -85,9 +85,6 @@ struct shg_handle {
     hipEvent_t fork_ev = nullptr, join_ev[3] = {};
     shg::Arena arena;
     std::map<int, double2*> tw64;
-    void* blas = nullptr;         // rocBLAS handle (heat solver), created lazily
-    void* blaslt = nullptr;       // hipBLASLt handle (heat solver's z transforms), created lazily
-    std::map<std::string, void*> lt_plans;  // hipBLASLt descriptors and algorithm per GEMM shape
     std::map<std::string, shg::HeatGeom*> heat_cache;  // direct-solver tables per geometry
     double* d_scalars = nullptr;  // reduction slots
     double* d_partials = nullptr;
