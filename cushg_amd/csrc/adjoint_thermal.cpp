@@ -19,10 +19,11 @@
 // five steps (Walker & Ni, SIAM J. Numer. Anal. 49, 1715 (2011)): the next w
 // is F(w_k) - sum_i g_i (dW_i + dF_i), g the least-squares fit of the
 // residual f_k = F(w_k) - w_k by the differences dF_i of earlier residuals
-// (dW_i of earlier iterates).  The iteration stops when the gradient changes
-// by less than 1e-5 of its size between sweeps (FP32-field forms, whose
-// rounding leaves ~3e-7; the error left is about a tenth of the last change)
-// or 1e-10 (FP64 forms), or after 2 max_outer sweeps.
+// (dW_i of earlier iterates).  The iteration stops when the gradient's
+// error -- its change between sweeps, or, once the changes contract by r <
+// 1/2 a sweep, the geometric-tail estimate r / (1 - r) times the last change
+// -- is below 1e-5 of its size (FP32-field forms, whose rounding leaves
+// ~3e-7) or 1e-10 (FP64 forms), or after 2 max_outer sweeps.
 //
 // The march is the FP32-field / FP64-arithmetic fused march (FP64 fields
 // for the FP64 forms).  Its last forward pass keeps every slice's
@@ -322,6 +323,8 @@ int run_adjoint_thermal(shg_handle* h, const shg_params* p, int ck, double* d_et
     };
     SHG_HIP(hipMemsetAsync(w, 0, sizeof(double) * NV, s1));
     int nhist = 0, hslot = 0;
+    double Gram[kMix][kMix] = {};  // dF_i . dF_j of the live history slots
+    double dprev = 0;           // the gradient's change in the previous sweep
     int it = 0;
     for (;; it++) {
         AdjThermal th{};
@@ -440,14 +443,24 @@ int run_adjoint_thermal(shg_handle* h, const shg_params* p, int ck, double* d_et
             set_error("non-finite gradient");
             return SHG_ENAN;
         }
-        if ((it > 0 && dmax <= tol * gmax) || it + 1 >= max_it) break;
+        // stop when the change is below tol, or when the error left, estimated
+        // from the observed contraction r of the last two changes as
+        // r / (1 - r) times the last change, is (r < 1/2, from the third sweep)
+        const double r = it >= 2 && dprev > 0 ? dmax / dprev : 1.0;
+        const bool settled = r < 0.5 && r / (1.0 - r) * dmax <= tol * gmax;
+        if ((it > 0 && (dmax <= tol * gmax || settled)) || it + 1 >= max_it) break;
+        dprev = dmax;
         // Anderson step: f = F(w) - w; history of (w - w_prev, f - f_prev)
         if (int rc = comb(fcur, {{1.0, Fw}, {-1.0, w}})) return rc;
         if (it > 0) {
             if (int rc = comb(dF[hslot], {{1.0, fcur}, {-1.0, fprev}})) return rc;
             if (int rc = comb(dW[hslot], {{1.0, w}, {-1.0, wprev}})) return rc;
-            hslot = (hslot + 1) % kMix;
             nhist = std::min(nhist + 1, kMix);
+            for (int j = 0; j < nhist; j++) {  // the new slot's row of the Gram matrix
+                if (int rc = dot(dF[hslot], dF[j], &Gram[hslot][j])) return rc;
+                Gram[j][hslot] = Gram[hslot][j];
+            }
+            hslot = (hslot + 1) % kMix;
         }
         std::swap(fcur, fprev);
         std::swap(w, wprev);  // w is rewritten below
@@ -460,10 +473,7 @@ int run_adjoint_thermal(shg_handle* h, const shg_params* p, int ck, double* d_et
         double Gm[kMix][kMix], rhs[kMix], gam[kMix];
         for (int i = 0; i < nhist; i++) {
             if (int rc = dot(dF[i], fprev, &rhs[i])) return rc;
-            for (int j = 0; j <= i; j++) {
-                if (int rc = dot(dF[i], dF[j], &Gm[i][j])) return rc;
-                Gm[j][i] = Gm[i][j];
-            }
+            for (int j = 0; j < nhist; j++) Gm[i][j] = Gram[i][j];
         }
         double tr = 0;
         for (int i = 0; i < nhist; i++) tr += Gm[i][i];

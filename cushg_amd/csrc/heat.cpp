@@ -534,6 +534,31 @@ static int axis_mfma(shg_handle* h, int axis, bool fwd, const DirectTables& dt, 
     return SHG_OK;
 }
 
+// a <- S^-1 a in FP32 on the matrix cores (fold, the six transforms around
+// the spectral division), leaving the result folded as launch_fd_update
+// expects; b is scratch.  Max|.| slots: 0 the folded input, 1-2 after the x
+// and y transforms, 3 after the division, 4-5 after the inverse z and y
+// transforms.
+static int inverse_fp32(shg_handle* h, const DirectTables& dt, float*& a, float*& b, int nx, int nyu, int nz) {
+    const bool fx = dt.fold[0], fz = dt.fold[2];
+    unsigned* m = dt.amax;
+    SHG_HIP(launch_zero_u32(h->stream, m, 8));
+    if (fx || fz) {
+        SHG_HIP(launch_fd_fold<float>(h->stream, a, b, nx, nyu, nz, fx, fz, m + 0));
+        std::swap(a, b);
+    } else {
+        SHG_HIP(launch_absmax(h->stream, a, (long)nx * nyu * nz, m + 0));
+    }
+    if (int rc = axis_mfma(h, 0, true, dt, a, b, nx, nyu, nz, m + 0, m + 1)) return rc;
+    if (int rc = axis_mfma(h, 1, true, dt, b, a, nx, nyu, nz, m + 1, m + 2)) return rc;
+    if (int rc = axis_mfma(h, 2, true, dt, a, b, nx, nyu, nz, m + 2, nullptr)) return rc;
+    SHG_HIP(launch_fd_divide<float>(h->stream, b, dt.ax, nx, nyu, nz, m + 3));
+    if (int rc = axis_mfma(h, 2, false, dt, b, a, nx, nyu, nz, m + 3, m + 4)) return rc;
+    if (int rc = axis_mfma(h, 1, false, dt, a, b, nx, nyu, nz, m + 4, m + 5)) return rc;
+    if (int rc = axis_mfma(h, 0, false, dt, b, a, nx, nyu, nz, m + 5, nullptr)) return rc;
+    return SHG_OK;
+}
+
 // correction pass: T += W^-1/2 S^-1 u (u from the residual pass); returns max|update|
 template <typename R>
 static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectTables& dt, double* max_corr) {
@@ -542,24 +567,8 @@ static int correct_pass(shg_handle* h, const HeatCoef& c, HeatState& hs, DirectT
     R* a = static_cast<R*>(dt.u);
     R* b = static_cast<R*>(dt.v);
     if constexpr (sizeof(R) == 4) {
-        // slots: 0 the folded input, 1-2 after the x and y transforms,
-        // 3 after the division, 4-5 after the inverse z and y transforms
-        unsigned* m = dt.amax;
         ProfScope ps(h, "heat_fd_gemm");
-        SHG_HIP(launch_zero_u32(h->stream, m, 8));
-        if (fx || fz) {
-            SHG_HIP(launch_fd_fold<R>(h->stream, a, b, nx, nyu, nz, fx, fz, m + 0));
-            std::swap(a, b);
-        } else {
-            SHG_HIP(launch_absmax(h->stream, a, (long)nx * nyu * nz, m + 0));
-        }
-        if (int rc = axis_mfma(h, 0, true, dt, a, b, nx, nyu, nz, m + 0, m + 1)) return rc;
-        if (int rc = axis_mfma(h, 1, true, dt, b, a, nx, nyu, nz, m + 1, m + 2)) return rc;
-        if (int rc = axis_mfma(h, 2, true, dt, a, b, nx, nyu, nz, m + 2, nullptr)) return rc;
-        SHG_HIP(launch_fd_divide<R>(h->stream, b, dt.ax, nx, nyu, nz, m + 3));
-        if (int rc = axis_mfma(h, 2, false, dt, b, a, nx, nyu, nz, m + 3, m + 4)) return rc;
-        if (int rc = axis_mfma(h, 1, false, dt, a, b, nx, nyu, nz, m + 4, m + 5)) return rc;
-        if (int rc = axis_mfma(h, 0, false, dt, b, a, nx, nyu, nz, m + 5, nullptr)) return rc;
+        if (int rc = inverse_fp32(h, dt, a, b, nx, nyu, nz)) return rc;
     } else {
         ProfScope ps(h, "heat_fd_gemm");
         if (fx || fz) {
@@ -687,10 +696,25 @@ int heat_adjoint_solve(shg_handle* h, const shg_params* p, HeatState& hs, const 
     }
     const int nx = c.nxu, nyu = c.nyu, nz = hs.nzh;
     const bool fx = g->fold[0], fz = g->fold[2];
+    ProfScope ps(h, "heat_adjoint_solve");
+    const DirectTables* dt = static_cast<const DirectTables*>(hs.solver);
+    if (dt && !dt->fp64) {
+        // FP32-field forms: one FP32 solve (relative error ~1e-6, below the
+        // forms' own rounding of the gradient) on the matrix cores
+        float* a = reinterpret_cast<float*>(u64);
+        float* b = reinterpret_cast<float*>(v64);
+        SHG_HIP(launch_adj_heat_in<float>(h->stream, c, g->ax, mu, a));
+        if (int rc = inverse_fp32(h, *dt, a, b, nx, nyu, nz)) return rc;
+        SHG_HIP(hipMemsetAsync(nu, 0, sizeof(double) * (size_t)c.nx * c.ny * nz, h->stream));
+        int nb = 0;
+        SHG_HIP(launch_fd_update<float>(h->stream, c, g->ax, a, nu, hs.part, fx, fz, &nb));
+        SHG_HIP(launch_adj_heat_out(h->stream, c, g->ax, mu, nu, w, hs.part));
+        SHG_HIP(launch_reduce(h->stream, hs.part, heat_grid_blocks(c), false, tp));
+        return SHG_OK;
+    }
     double* a = u64;
     double* b = v64;
-    ProfScope ps(h, "heat_adjoint_solve");
-    SHG_HIP(launch_adj_heat_in(h->stream, c, g->ax, mu, a));
+    SHG_HIP(launch_adj_heat_in<double>(h->stream, c, g->ax, mu, a));
     if (fx || fz) {
         SHG_HIP(launch_fd_fold<double>(h->stream, a, b, nx, nyu, nz, fx, fz));
         std::swap(a, b);

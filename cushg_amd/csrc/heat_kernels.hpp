@@ -106,7 +106,8 @@ hipError_t launch_max(hipStream_t st, const double* a, long n, double* part, int
 // temperature's derivative (at most heat_grid_blocks(c)); dL/dT per voxel
 // from the adjoint row passes' accumulators (into gnp), plus tc at the
 // centre voxel of slab 0
-hipError_t launch_adj_heat_in(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu, double* u);
+template <typename R>
+hipError_t launch_adj_heat_in(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu, R* u);
 hipError_t launch_adj_heat_out(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu,
                                const double* nu, double* w, double* part);
 hipError_t launch_adj_heat_grad(hipStream_t st, const double* T, double* gnp, const double* gns, const double* gA,

@@ -392,8 +392,9 @@ __device__ __forceinline__ bool heat_fixed(const HeatCoef& c, int x, int y) {
 
 // u = W^-1/2 mu on the unknown block, in the layout k_heat_resid writes:
 // the right-hand side of the transposed (= the same, symmetric) solve.
+template <typename R>
 __global__ __launch_bounds__(256) void k_adj_heat_in(HeatCoef c, HeatAxes a, const double* __restrict__ mu,
-                                                     double* __restrict__ u) {
+                                                     R* __restrict__ u) {
     const int xu = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     if (xu >= c.nxu || j >= c.nyu) return;
     const int x = xu + c.x0, y = j + 1;
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(256) void k_adj_heat_in(HeatCoef c, HeatAxes a, con
     const double wxy = a.wx[xu] * a.wy[j];
     for (int z = zr.z0; z < zr.z1; z++)
         u[((size_t)z * c.nyu + j) * c.nxu + xu] =
-            mu[((size_t)z * c.ny + y) * c.nx + x] * rsqrt(wxy * a.wz[z]);
+            (R)(mu[((size_t)z * c.ny + y) * c.nx + x] * rsqrt(wxy * a.wz[z]));
 }
 
 // After nu = A^-1 mu (A the finite-volume operator of the steady state):
@@ -575,11 +576,16 @@ hipError_t launch_heat_dk(hipStream_t st, const double* T, double* dk, float2* n
     return hipGetLastError();
 }
 
-hipError_t launch_adj_heat_in(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu, double* u) {
+template <typename R>
+hipError_t launch_adj_heat_in(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu, R* u) {
     const dim3 pg = plane_grid(c);
-    hipLaunchKernelGGL(k_adj_heat_in, dim3((c.nxu + 63) / 64, (c.nyu + 3) / 4, pg.z), dim3(64, 4), 0, st, c, a, mu, u);
+    hipLaunchKernelGGL((k_adj_heat_in<R>), dim3((c.nxu + 63) / 64, (c.nyu + 3) / 4, pg.z), dim3(64, 4), 0, st, c, a,
+                       mu, u);
     return hipGetLastError();
 }
+template hipError_t launch_adj_heat_in<float>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*, float*);
+template hipError_t launch_adj_heat_in<double>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*,
+                                               double*);
 
 hipError_t launch_adj_heat_out(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* mu,
                                const double* nu, double* w, double* part) {
