@@ -127,16 +127,24 @@ def adjoint_thermal_cost(h, batch: int) -> dict:
         for q in ps:
             q.precision = abi.PREC_FP32_STORAGE
         st = optimize.ascent_step(h, ps)
+        t = time.perf_counter()
+        for q in ps:
+            h.run(q, fields=False)
+        ts = time.perf_counter() - t
         rc, rb, _ = h.run_batch(ps)
         t = time.perf_counter()
         rc, rb, _ = h.run_batch(ps)
         tb = time.perf_counter() - t
         gains = [s.eta1 / s.eta0 - 1 for s in st.steps]
         out["ascent_step"] = {"instances": batch, "forward_plus_adjoint_s": round(st.grad_wall_s, 2),
+                              "forward_s": round(ts, 2), "ratio": round(st.grad_wall_s / ts, 3),
+                              "ratio_basis": "the instances' forward + adjoint calls over their forward calls, "
+                                             "both one instance at a time",
                               "step_s": round(st.wall_s, 2), "batched_forward_s": round(tb, 2),
                               "accepted": sum(s.accepted for s in st.steps),
                               "eta_gain_rel": {"min": min(gains), "max": max(gains)}}
-        print(f"bench: ascent step over {batch}: {st.wall_s:.1f} s", file=sys.stderr, flush=True)
+        print(f"bench: ascent step over {batch}: {st.wall_s:.1f} s (forward + adjoint {st.grad_wall_s:.1f} s, "
+              f"forward {ts:.1f} s, ratio {st.grad_wall_s / ts:.2f})", file=sys.stderr, flush=True)
     return out
 
 

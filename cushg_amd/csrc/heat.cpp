@@ -200,6 +200,7 @@ struct DirectTables {
     const _Float16* qs[3] = {nullptr, nullptr, nullptr};
     int qe[3] = {}, rpad[3] = {}, kpad[3] = {};
     unsigned* amax = nullptr;
+    unsigned* apart = nullptr;  // per-workgroup maxima of those reductions
 };
 
 // Per-geometry tables of the direct solver (axis widths, stiffness
@@ -337,6 +338,7 @@ size_t heat_bytes(const shg_params* p) {
     b += 2 * al256(rs * nu);                           // u, v
     const size_t nb = (size_t)((p->nx + 63) / 64) * ((p->ny + 3) / 4) * nzh;
     b += al256(8 * nb) + al256(8 * 8) + al256(4 * 8);  // partials, slots, max|u| slots
+    b += al256(4 * (size_t)heat_max_partials(c.nxu, c.nyu, (int)nzh));  // their per-workgroup maxima
     b += al256(8 * (size_t)p->nz);                     // poling
     b += 4096;
     return b;
@@ -375,7 +377,8 @@ int heat_alloc(shg_handle* h, const shg_params* p, HeatState& hs) {
     dt->u = f64 ? (void*)A.carve<double>(nu) : (void*)A.carve<float>(nu);
     dt->v = f64 ? (void*)A.carve<double>(nu) : (void*)A.carve<float>(nu);
     dt->amax = A.carve<unsigned>(8);
-    if (!hs.T || !hs.Q || !hs.red || !dt->u || !dt->v || !dt->amax) {
+    dt->apart = A.carve<unsigned>((size_t)heat_max_partials(c.nxu, c.nyu, hs.nzh));
+    if (!hs.T || !hs.Q || !hs.red || !dt->u || !dt->v || !dt->amax || !dt->apart) {
         set_error("arena exhausted (heat buffers)");
         return SHG_ENOMEM;
     }
@@ -499,6 +502,7 @@ static int axis_mfma(shg_handle* h, int axis, bool fwd, const DirectTables& dt, 
     g.ql = base + nb * blk;
     g.umax = umax;
     g.cmax = cmax;
+    g.cpart = dt.apart;
     g.qpad = dt.kpad[axis];
     g.qe = dt.qe[axis];
     int batch = nb;
@@ -544,15 +548,15 @@ static int inverse_fp32(shg_handle* h, const DirectTables& dt, float*& a, float*
     unsigned* m = dt.amax;
     SHG_HIP(launch_zero_u32(h->stream, m, 8));
     if (fx || fz) {
-        SHG_HIP(launch_fd_fold<float>(h->stream, a, b, nx, nyu, nz, fx, fz, m + 0));
+        SHG_HIP(launch_fd_fold<float>(h->stream, a, b, nx, nyu, nz, fx, fz, dt.apart, m + 0));
         std::swap(a, b);
     } else {
-        SHG_HIP(launch_absmax(h->stream, a, (long)nx * nyu * nz, m + 0));
+        SHG_HIP(launch_absmax(h->stream, a, (long)nx * nyu * nz, dt.apart, m + 0));
     }
     if (int rc = axis_mfma(h, 0, true, dt, a, b, nx, nyu, nz, m + 0, m + 1)) return rc;
     if (int rc = axis_mfma(h, 1, true, dt, b, a, nx, nyu, nz, m + 1, m + 2)) return rc;
     if (int rc = axis_mfma(h, 2, true, dt, a, b, nx, nyu, nz, m + 2, nullptr)) return rc;
-    SHG_HIP(launch_fd_divide<float>(h->stream, b, dt.ax, nx, nyu, nz, m + 3));
+    SHG_HIP(launch_fd_divide<float>(h->stream, b, dt.ax, nx, nyu, nz, dt.apart, m + 3));
     if (int rc = axis_mfma(h, 2, false, dt, b, a, nx, nyu, nz, m + 3, m + 4)) return rc;
     if (int rc = axis_mfma(h, 1, false, dt, a, b, nx, nyu, nz, m + 4, m + 5)) return rc;
     if (int rc = axis_mfma(h, 0, false, dt, b, a, nx, nyu, nz, m + 5, nullptr)) return rc;

@@ -41,7 +41,8 @@ struct HeatAxes {
 // qh, ql: the eigenbasis in the direction asked for, split into FP16 hi and
 // lo parts of 2^qe q, [row][k] with rows padded to a multiple of 128 and k
 // (row stride qpad) to a multiple of 32; umax: max|u| as float bits; cmax
-// (optional): receives max|C| (atomic max on float bits, cleared before).
+// (optional): receives max|C| as float bits (cleared before), the
+// per-workgroup maxima staged in cpart.
 struct HeatGemm {
     const float* u;
     float* c;
@@ -49,6 +50,7 @@ struct HeatGemm {
     const _Float16* ql;
     const unsigned* umax;
     unsigned* cmax;
+    unsigned* cpart;
     long ldu, ldc, su, sc, sq, nbig;
     int n, qpad, qe, kindB;
 };
@@ -69,7 +71,9 @@ struct HeatGemm64 {
     int n;
 };
 hipError_t launch_heat_gemm64(hipStream_t st, const HeatGemm64& g, int batch);
-hipError_t launch_absmax(hipStream_t st, const float* a, long n, unsigned* out);
+// max |a| over n floats into *out (float bits; *out 0 before), per-workgroup
+// maxima staged in part (1024 entries)
+hipError_t launch_absmax(hipStream_t st, const float* a, long n, unsigned* part, unsigned* out);
 hipError_t launch_zero_u32(hipStream_t st, unsigned* p, int n);
 
 HeatCoef heat_coef(const shg_params* p, int nzh);
@@ -80,15 +84,23 @@ hipError_t launch_heat_sweep(hipStream_t st, const HeatCoef& c, const double* Ti
 template <typename R>
 hipError_t launch_heat_resid(hipStream_t st, const HeatCoef& c, const HeatAxes& a, const double* T, const float* Q,
                              R* u, double* part_sq);
-// amax (optional, FP32 only): atomic max of |v| after the division, as float bits
+// part, slot (optional, FP32 only): max |v| after the division, as float
+// bits, into *slot (per-workgroup maxima staged in part, at least
+// heat_max_partials entries; *slot must hold 0 or a smaller value before)
 template <typename R>
 hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz,
-                            unsigned* amax = nullptr);
+                            unsigned* part = nullptr, unsigned* slot = nullptr);
 // even/odd fold of the x and/or z axis of u (in -> out; not in place);
-// amax as for launch_fd_divide, of the output
+// part, slot as for launch_fd_divide, of the output
 template <typename R>
 hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz,
-                          unsigned* amax = nullptr);
+                          unsigned* part = nullptr, unsigned* slot = nullptr);
+// entries of the part buffer that the max reductions of the heat solve's
+// FP32 transforms need at most on an nxu x nyu x nz unknown block
+long heat_max_partials(int nxu, int nyu, int nz);
+long heat_gemm_max_blocks(int nxu, int nyu, int nz);
+// *slot = max(*slot, max of part[0..n))
+hipError_t launch_max_parts(hipStream_t st, const unsigned* part, long n, unsigned* slot);
 // T += W^-1/2 u, u folded along x / z as launch_fd_fold leaves it; writes
 // *nblocks partial maxima (at most heat_grid_blocks(c))
 template <typename R>

@@ -175,20 +175,24 @@ __global__ __launch_bounds__(256) void k_heat_resid(HeatCoef c, HeatAxes a, cons
 // are multiplied by the FP32 reciprocal of the FP64 sum (within an ulp or two
 // of the division, the precision of the FP32 transforms around it); FP64
 // data are divided.
-// workgroup max of m (256 threads, every thread calling) into *amax
-__device__ __forceinline__ void block_amax(float m, unsigned* amax) {
+// workgroup max of m (256 threads, every thread calling), as float bits,
+// into part[linear workgroup id] (no atomics: a single-address atomic per
+// workgroup serialises ~1e5 workgroups)
+__device__ __forceinline__ void block_max_part(float m, unsigned* part) {
     __shared__ float wm[4];
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     if ((t & 63) == 0) wm[t >> 6] = m;
     __syncthreads();
-    if (t == 0) atomicMax(amax, __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
+    if (t == 0)
+        part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+            __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3])));
 }
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_divide(R* __restrict__ v, const double* __restrict__ lx,
                                                    const double* __restrict__ ly, const double* __restrict__ lz,
-                                                   int nxu, int nyu, int nz, unsigned* __restrict__ amax) {
+                                                   int nxu, int nyu, int nz, unsigned* __restrict__ part) {
     const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
     float m = 0.0f;
     if (x < nxu) {
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(256) void k_fd_divide(R* __restrict__ v, const doub
             }
         }
     }
-    if (amax) block_amax(m, amax);
+    if (part) block_max_part(m, part);
 }
 
 // Even/odd fold of the symmetric axes of u[nz][nyu][nxu]: along a folded
@@ -219,7 +223,7 @@ constexpr double kRsqrt2 = 0.70710678118654752440;
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_fd_fold(const R* __restrict__ in, R* __restrict__ out, int nxu, int nyu,
-                                                 int nz, int fx, int fz, unsigned* __restrict__ amax) {
+                                                 int nz, int fx, int fz, unsigned* __restrict__ part) {
     const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
     const int xu = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
     float m = 0.0f;
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(256) void k_fd_fold(const R* __restrict__ in, R* __
             put(at(z, xu), in[at(z, xu)]);
         }
     }
-    if (amax) block_amax(m, amax);
+    if (part) block_max_part(m, part);
 }
 
 // T += W^-1/2 u (the correction), u unfolded first along the folded axes
@@ -517,27 +521,45 @@ template hipError_t launch_heat_resid<float>(hipStream_t, const HeatCoef&, const
 template hipError_t launch_heat_resid<double>(hipStream_t, const HeatCoef&, const HeatAxes&, const double*,
                                               const float*, double*, double*);
 
+static dim3 divide_grid(int nxu, int nyu, int nz) {
+    return dim3((nxu + 255) / 256, nyu, std::max(1, std::min(nz, 16)));
+}
+static dim3 fold_grid(int nxu, int nyu, int nz, bool fx, bool fz) {
+    const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
+    return dim3((hx + 63) / 64, (nyu + 3) / 4, hz);
+}
+static long blocks(dim3 g) { return (long)g.x * g.y * g.z; }
+
+long heat_max_partials(int nxu, int nyu, int nz) {
+    return std::max({blocks(divide_grid(nxu, nyu, nz)), blocks(fold_grid(nxu, nyu, nz, false, false)),
+                     heat_gemm_max_blocks(nxu, nyu, nz), 1024L});
+}
+
 template <typename R>
-hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz, unsigned* amax) {
-    const int zc = std::max(1, std::min(nz, 16));
-    hipLaunchKernelGGL((k_fd_divide<R>), dim3((nxu + 255) / 256, nyu, zc), dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu,
-                       nyu, nz, amax);
+hipError_t launch_fd_divide(hipStream_t st, R* v, const HeatAxes& a, int nxu, int nyu, int nz, unsigned* part,
+                            unsigned* slot) {
+    const dim3 grid = divide_grid(nxu, nyu, nz);
+    hipLaunchKernelGGL((k_fd_divide<R>), grid, dim3(256), 0, st, v, a.lx, a.ly, a.lz, nxu, nyu, nz, part);
+    if (part) return launch_max_parts(st, part, blocks(grid), slot);
     return hipGetLastError();
 }
-template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int, unsigned*);
-template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxes&, int, int, int, unsigned*);
+template hipError_t launch_fd_divide<float>(hipStream_t, float*, const HeatAxes&, int, int, int, unsigned*,
+                                            unsigned*);
+template hipError_t launch_fd_divide<double>(hipStream_t, double*, const HeatAxes&, int, int, int, unsigned*,
+                                             unsigned*);
 
 template <typename R>
 hipError_t launch_fd_fold(hipStream_t st, const R* in, R* out, int nxu, int nyu, int nz, bool fx, bool fz,
-                          unsigned* amax) {
-    const int hx = fx ? nxu / 2 : nxu, hz = fz ? nz / 2 : nz;
-    hipLaunchKernelGGL((k_fd_fold<R>), dim3((hx + 63) / 64, (nyu + 3) / 4, hz), dim3(64, 4), 0, st, in, out, nxu, nyu,
-                       nz, fx ? 1 : 0, fz ? 1 : 0, amax);
+                          unsigned* part, unsigned* slot) {
+    const dim3 grid = fold_grid(nxu, nyu, nz, fx, fz);
+    hipLaunchKernelGGL((k_fd_fold<R>), grid, dim3(64, 4), 0, st, in, out, nxu, nyu, nz, fx ? 1 : 0, fz ? 1 : 0, part);
+    if (part) return launch_max_parts(st, part, blocks(grid), slot);
     return hipGetLastError();
 }
-template hipError_t launch_fd_fold<float>(hipStream_t, const float*, float*, int, int, int, bool, bool, unsigned*);
+template hipError_t launch_fd_fold<float>(hipStream_t, const float*, float*, int, int, int, bool, bool, unsigned*,
+                                          unsigned*);
 template hipError_t launch_fd_fold<double>(hipStream_t, const double*, double*, int, int, int, bool, bool,
-                                           unsigned*);
+                                           unsigned*, unsigned*);
 
 // The grid keeps the block count of plane_grid (the size of the partials
 // array) at most: the z chunks take the x blocks a fold saves.

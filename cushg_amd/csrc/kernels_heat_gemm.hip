@@ -27,6 +27,8 @@
 // of one row or column) is one 16-byte read.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "heat_kernels.hpp"
 
 namespace shg {
@@ -218,7 +220,9 @@ __global__ __launch_bounds__(256) void k_heat_gemm(HeatGemm g) {
         for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
         if (lane == 0) wmax[w] = m;
         __syncthreads();
-        if (t == 0) atomicMax(g.cmax, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
+        if (t == 0)
+            g.cpart[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+                __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
     }
 }
 
@@ -304,15 +308,27 @@ hipError_t launch_heat_gemm64(hipStream_t st, const HeatGemm64& g, int batch) {
     return hipGetLastError();
 }
 
-// max |a| over n floats into *out (as float bits; *out must start at 0)
-__global__ __launch_bounds__(256) void k_absmax(const float* __restrict__ a, long n, unsigned* __restrict__ out) {
+// max |a| over n floats: per-workgroup maxima into part (float bits)
+__global__ __launch_bounds__(256) void k_absmax(const float* __restrict__ a, long n, unsigned* __restrict__ part) {
     __shared__ float wm[4];
     float m = 0.0f;
     for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) m = fmaxf(m, fabsf(a[i]));
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
+    if (threadIdx.x == 0) part[blockIdx.x] = __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3])));
+}
+
+// *slot = max(*slot, part[0..n)): a few workgroups, one atomic each
+__global__ __launch_bounds__(256) void k_max_parts(const unsigned* __restrict__ part, long n,
+                                                   unsigned* __restrict__ slot) {
+    __shared__ unsigned wm[4];
+    unsigned m = 0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) m = max(m, part[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(slot, max(max(wm[0], wm[1]), max(wm[2], wm[3])));
 }
 
 // the split eigenbasis of one axis in one direction: block b, row m, k of
@@ -338,6 +354,20 @@ __global__ void k_zero_u32(unsigned* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
+hipError_t launch_max_parts(hipStream_t st, const unsigned* part, long n, unsigned* slot) {
+    const long nb = std::min(64L, (n + 255) / 256);
+    hipLaunchKernelGGL(k_max_parts, dim3((unsigned)std::max(1L, nb)), dim3(256), 0, st, part, n, slot);
+    return hipGetLastError();
+}
+
+static long gemm_tiles(long rows, long cols) { return ((rows + kTI - 1) / kTI) * ((cols + kTJ - 1) / kTJ); }
+
+long heat_gemm_max_blocks(int nxu, int nyu, int nz) {
+    // unfolded lengths bound the folded ones (two blocks of half the length)
+    return std::max({gemm_tiles((long)nyu * nz, nxu) * 2, gemm_tiles(nyu, nxu) * nz,
+                     gemm_tiles(nz, (long)nxu * nyu) * 2});
+}
+
 hipError_t launch_heat_gemm(hipStream_t st, const HeatGemm& g, int batch) {
     const long ti = ((g.kindB ? g.nbig : g.n) + kTI - 1) / kTI;
     const long tj = ((g.kindB ? g.n : g.nbig) + kTJ - 1) / kTJ;
@@ -345,12 +375,13 @@ hipError_t launch_heat_gemm(hipStream_t st, const HeatGemm& g, int batch) {
         hipLaunchKernelGGL(k_heat_gemm<true>, dim3((unsigned)ti, (unsigned)tj, batch), dim3(256), 0, st, g);
     else
         hipLaunchKernelGGL(k_heat_gemm<false>, dim3((unsigned)tj, (unsigned)ti, batch), dim3(256), 0, st, g);
+    if (g.cmax) return launch_max_parts(st, g.cpart, ti * tj * batch, g.cmax);
     return hipGetLastError();
 }
 
-hipError_t launch_absmax(hipStream_t st, const float* a, long n, unsigned* out) {
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, st, a, n, out);
-    return hipGetLastError();
+hipError_t launch_absmax(hipStream_t st, const float* a, long n, unsigned* part, unsigned* out) {
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, st, a, n, part);
+    return launch_max_parts(st, part, 1024, out);
 }
 
 hipError_t launch_split_basis(hipStream_t st, const double* q, int n, int nb, bool inv, int qe, _Float16* hi,

@@ -30,7 +30,7 @@ EXPORTS = (
     "shg_refractive_index", "shg_delta_k", "shg_validate", "shg_last_error",
     "shg_handle_create", "shg_handle_destroy", "shg_run", "shg_run_traced", "shg_run_batch", "shg_adjoint",
     "shg_cli_main", "shg_last_kernel_times", "shg_handle_set_profiling", "shg_set_plan",
-    "shg_get_plan", "shg_autotune", "shg_set_graph_steps", "shg_run_noalloc",
+    "shg_get_plan", "shg_autotune", "shg_set_graph_steps", "shg_run_noalloc", "shg_fft2d",
 )
 
 _lib = None
@@ -83,6 +83,8 @@ def lib():
     L.shg_autotune.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.shg_set_graph_steps.argtypes = [H, C.c_int32]
     L.shg_run_noalloc.argtypes = [H, P, R]
+    L.shg_fft2d.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                            C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -209,6 +211,17 @@ class Handle:
         r = ShgResult()
         _check(lib().shg_adjoint(self._h, C.byref(p), checkpoint_every, _dp(g), C.byref(r)))
         return r, g
+
+    def fft2d(self, x: np.ndarray, precision: int = abi.PREC_FP64_FUSED, inverse: bool = False) -> np.ndarray:
+        """shg_fft2d: the library's unnormalised 2D DFT of x[ny, nx] (complex)."""
+        a = np.ascontiguousarray(x, dtype=np.complex128)
+        if a.ndim != 2:
+            raise ValueError("x must be 2-D [ny, nx]")
+        ny, nx = a.shape
+        out = np.empty_like(a)
+        _check(lib().shg_fft2d(self._h, nx, ny, precision, int(inverse), _dp(a.view(np.float64)),
+                               _dp(out.view(np.float64))))
+        return out
 
     def set_profiling(self, on: bool):
         _check(lib().shg_handle_set_profiling(self._h, int(on)))

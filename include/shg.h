@@ -333,6 +333,17 @@ SHG_API int shg_autotune(shg_handle* h, int32_t n, int32_t precision, int32_t ba
  * per-slice host loop with its synchronisations (src/headers/Solver.h:276-308). */
 SHG_API int shg_set_graph_steps(shg_handle* h, int32_t steps);
 
+/* The library's 2D DFT on a caller's array: ny rows of nx complex points
+ * (interleaved doubles, x fastest), unnormalised, forward exp(-2 pi i jk/N)
+ * (inverse != 0: exp(+2 pi i jk/N)).  The same transform code, twiddle
+ * tables and arithmetic as the fused passes that replace cufftExecC2C
+ * (src/headers/Solver.h:159-183): FP64 for SHG_PREC_FP64_PLAIN/_FUSED; FP32
+ * points with FP64 twiddle products for the FP32 forms (inputs rounded to
+ * FP32).  nx and ny as shg_fft_size_supported; allocates its device buffer
+ * (SHG_ENOMEM on failure). */
+SHG_API int shg_fft2d(shg_handle* h, int32_t nx, int32_t ny, int32_t precision, int32_t inverse, const double* in,
+                      double* out);
+
 /* ---------------------------------------------------------------------
  * Zero-allocation run.  No counterpart in the reference: this entry point
  * is for callers that cannot allocate device memory (code under HIP graph
