@@ -97,6 +97,7 @@ class ShgResult(C.Structure):
         ("outer_iters", C.c_int32), ("status", C.c_int32),
         ("pump_out", C.POINTER(C.c_double)), ("signal_out", C.POINTER(C.c_double)),
         ("temperature_out", C.POINTER(C.c_double)),
+        ("adjoint_sweeps", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 

@@ -295,6 +295,7 @@ int run_adjoint(shg_handle* h, const shg_params* p, int ck, double* d_eta, shg_r
     res->eta = p->power_W > 0 ? res->P_sh_W / p->power_W : 0.0;
     res->steps_done = nz;
     res->outer_iters = 1;
+    res->adjoint_sweeps = 1;
     res->status = SHG_OK;
     if (!std::isfinite(res->eta)) {
         set_error("non-finite field power");

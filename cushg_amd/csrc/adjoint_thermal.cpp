@@ -515,6 +515,7 @@ int run_adjoint_thermal(shg_handle* h, const shg_params* p, int ck, double* d_et
             if (int rc = comb(w, {{1.0, w}, {-gam[i], dW[i]}, {-gam[i], dF[i]}})) return rc;
     }
     for (size_t i = 0; i < grad.size(); i++) d_eta[i] = grad[i];
+    res->adjoint_sweeps = it + 1;
     res->status = SHG_OK;
     res->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return SHG_OK;

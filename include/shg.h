@@ -177,6 +177,11 @@ typedef struct shg_result {
     double* pump_out;        /* 2*nx*ny doubles, interleaved */
     double* signal_out;      /* 2*nx*ny doubles, interleaved */
     double* temperature_out; /* nx*ny*nz_heat doubles (THERMAL) */
+
+    /* set by shg_adjoint only: reverse sweeps of the march it ran (1 without
+     * THERMAL; the fixed-point sweeps at the thermo-optic steady state with) */
+    int32_t adjoint_sweeps;
+    int32_t reserved0;
 } shg_result;
 
 /* Supported transverse sizes of the GPU FFT: 64, 128, 192, 256, 384, 512
