@@ -233,4 +233,123 @@ __device__ __forceinline__ void fft_dist(C (&v)[8], C* lds, const tw_t<C>* __res
     fft_multi<C, N, INV, 1>(reinterpret_cast<C(&)[1][8]>(v), lds, 0, tw, tp, act);
 }
 
+// Twiddles W^(j m), j = 1, 2, ..., as FP64 powers of the table entry
+// W^m = tw[m] (one table read per pass instead of one per point; each power
+// carries a relative error of about j FP64 ulps, far below the FP32 and FP64
+// rounding of the transform itself).  mul(x) multiplies by the next power.
+template <bool INV> struct TwPow {
+    double2 w, p;
+    __device__ __forceinline__ TwPow(const double2* __restrict__ tw, int m) {
+        w = tw[m];
+        if (INV) w.y = -w.y;
+        p = w;
+    }
+    __device__ __forceinline__ float2 mul(float2 x) {
+        const double xr = x.x, xi = x.y;
+        const float2 r = make_float2((float)(xr * p.x - xi * p.y), (float)(xr * p.y + xi * p.x));
+        p = cmul(p, w);
+        return r;
+    }
+    __device__ __forceinline__ double2 mul(double2 x) {
+        const double2 r = cmul(x, p);
+        p = cmul(p, w);
+        return r;
+    }
+};
+
+// Forward transform, a pointwise operation op(v) on the natural
+// distribution, inverse transform: the sequence of the column pass, in the
+// arithmetic of fft_multi (radices {8, M, 8}, the same butterflies and
+// products) but with fewer barriers and twiddle reads:
+//  * the middle pass writes each butterfly's outputs back into the slots it
+//    read (no barrier between its reads and writes), and the last pass reads
+//    them from there;
+//  * the inverse transform's first pass writes each thread's outputs into
+//    the slots that thread read in the forward transform's last pass, so no
+//    barrier separates the two transforms;
+//  * twiddles are FP64 powers of one table entry per pass (TwPow).
+// With tp = 8u + w, the physical slots (before padding) are:
+//   forward  pass 1 writes 8 tp + q; middle butterfly b reads/writes
+//            b + 64 j; pass 3 reads 8 j + w + 64 u
+//   inverse  pass 1 writes 8 q + w + 64 u; middle butterfly b reads/writes
+//            8 (b & 7) + (b >> 3) + 64 j; pass 3 reads 8 w + j + 64 u
+// (each a relabelling of the Stockham slots of fft_multi: the values and
+// the arithmetic are the same).  Two barriers per transform; none at the
+// end: a caller that touches `lds` afterwards must synchronise first.
+template <typename C, int N, typename Op>
+__device__ __forceinline__ void fft_fwd_op_inv(C (&v)[8], C* lds, const double2* __restrict__ tw, int tp, Op op) {
+    constexpr int TPR = FftShape<N>::TPR;
+    constexpr int M = FftShape<N>::M;
+    constexpr int PER = (64 + TPR - 1) / TPR;
+    const int w = tp & 7, u = tp >> 3;
+    // forward
+    dft8<C, false>(v);
+#pragma unroll
+    for (int q = 0; q < 8; q++) lds[fpad(8 * tp + q)] = v[q];
+    __syncthreads();
+    if constexpr (M > 1) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int b = tp + i * TPR;
+            if (PER * TPR == 64 || b < 64) {
+                C xs[M];
+                TwPow<false> t(tw, 8 * (b & 7));
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    const C a = lds[fpad(b + 64 * j)];
+                    xs[j] = j ? t.mul(a) : a;
+                }
+                dftR<C, false, M>(xs);
+#pragma unroll
+                for (int q = 0; q < M; q++) lds[fpad(b + 64 * q)] = xs[q];
+            }
+        }
+        __syncthreads();
+    }
+    {
+        TwPow<false> t(tw, tp);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const C a = lds[fpad(8 * j + w + 64 * u)];
+            v[j] = j ? t.mul(a) : a;
+        }
+        dft8<C, false>(v);
+    }
+    op(v);
+    // inverse
+    dft8<C, true>(v);
+#pragma unroll
+    for (int q = 0; q < 8; q++) lds[fpad(8 * q + w + 64 * u)] = v[q];
+    __syncthreads();
+    if constexpr (M > 1) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int b = tp + i * TPR;
+            if (PER * TPR == 64 || b < 64) {
+                const int s0 = 8 * (b & 7) + (b >> 3);
+                C xs[M];
+                TwPow<true> t(tw, 8 * (b & 7));
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    const C a = lds[fpad(s0 + 64 * j)];
+                    xs[j] = j ? t.mul(a) : a;
+                }
+                dftR<C, true, M>(xs);
+#pragma unroll
+                for (int q = 0; q < M; q++) lds[fpad(s0 + 64 * q)] = xs[q];
+            }
+        }
+        __syncthreads();
+    }
+    {
+        TwPow<true> t(tw, tp);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const C a = lds[fpad(8 * w + j + 64 * u)];
+            v[j] = j ? t.mul(a) : a;
+        }
+        dft8<C, true>(v);
+    }
+}
+
 }  // namespace shg

@@ -459,7 +459,8 @@ __global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const Mix
 }
 
 // Column pass of step g: column FFT -> * Prop -> column IFFT on COLS
-// adjacent columns of one field (blockIdx.y: 0 pump, 1 SH).
+// adjacent columns of one field (blockIdx.y: 0 pump, 1 SH), through
+// fft_fwd_op_inv (four workgroup barriers for the pair of transforms).
 // grid (ceil(max_nx/COLS), 2, n_instances), block (COLS, NY/8)
 // FP64 arithmetic: keep 4 waves per SIMD (<= 128 VGPRs), the most the
 // LDS footprint of the FP64 exchange allows anyway
@@ -498,17 +499,17 @@ void k_colpass(const MixedInst* __restrict__ insts, Step gs, const double2* __re
     C v[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) v[q] = from_store<C>(F[(size_t)(tp + q * TPC) * nx + x]);
-    fft_dist<C, NY, false>(v, my, tw, tp);
     // Prop = ex[x] ey[y], formed and applied in FP64 and rounded once to the
     // arithmetic type, so it carries no fixed representation error per mode.
     const double2 exv = ex[x];
+    fft_fwd_op_inv<C, NY>(v, my, tw, tp, [&](C (&a)[8]) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const double2 pr = cmul(exv, ey[tp + q * TPC]);
-        const double vr = v[q].x, vi = v[q].y;
-        v[q] = mk<C>((R)(vr * pr.x - vi * pr.y), (R)(vr * pr.y + vi * pr.x));
-    }
-    fft_dist<C, NY, true>(v, my, tw, tp);
+        for (int q = 0; q < 8; q++) {
+            const double2 pr = cmul(exv, ey[tp + q * TPC]);
+            const double vr = a[q].x, vi = a[q].y;
+            a[q] = mk<C>((R)(vr * pr.x - vi * pr.y), (R)(vr * pr.y + vi * pr.x));
+        }
+    });
     const uint32_t key = store_key(1, sh ? 1 : 0, g);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
