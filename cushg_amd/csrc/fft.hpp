@@ -257,66 +257,83 @@ template <bool INV> struct TwPow {
     }
 };
 
-// Forward transform, a pointwise operation op(v) on the natural
-// distribution, inverse transform: the sequence of the column pass, in the
-// arithmetic of fft_multi (radices {8, M, 8}, the same butterflies and
-// products) but with fewer barriers and twiddle reads:
+// One transform (either direction) in the arithmetic of fft_multi (radices
+// {8, M, 8}, the same butterflies and products) with fewer barriers and
+// twiddle reads:
 //  * the middle pass writes each butterfly's outputs back into the slots it
 //    read (no barrier between its reads and writes), and the last pass reads
 //    them from there;
-//  * the inverse transform's first pass writes each thread's outputs into
-//    the slots that thread read in the forward transform's last pass, so no
-//    barrier separates the two transforms;
 //  * twiddles are FP64 powers of one table entry per pass (TwPow).
-// With tp = 8u + w, the physical slots (before padding) are:
-//   forward  pass 1 writes 8 tp + q; middle butterfly b reads/writes
-//            b + 64 j; pass 3 reads 8 j + w + 64 u
-//   inverse  pass 1 writes 8 q + w + 64 u; middle butterfly b reads/writes
-//            8 (b & 7) + (b >> 3) + 64 j; pass 3 reads 8 w + j + 64 u
-// (each a relabelling of the Stockham slots of fft_multi: the values and
-// the arithmetic are the same).  Two barriers per transform; none at the
-// end: a caller that touches `lds` afterwards must synchronise first.
+// With tp = 8u + w, the physical slots (before padding) are: pass 1 writes
+// 8 tp + q; middle butterfly b reads/writes b + 64 j; pass 3 reads
+// 8 j + w + 64 u (a relabelling of fft_multi's Stockham slots: the values
+// and the arithmetic are the same).  Two barriers, and a third at the end
+// when END (the buffer may then be reused at once); without it, a caller
+// that writes `lds` afterwards must synchronise first, unless each thread
+// writes only the slots it read in pass 3 (as fft_fwd_op_inv does).  A
+// thread with act = false only takes part in the barriers.
+template <typename C, int N, bool INV, bool END>
+__device__ __forceinline__ void fft_pow(C (&v)[8], C* lds, const double2* __restrict__ tw, int tp, bool act = true) {
+    constexpr int TPR = FftShape<N>::TPR;
+    constexpr int M = FftShape<N>::M;
+    constexpr int PER = (64 + TPR - 1) / TPR;
+    const int w = tp & 7, u = tp >> 3;
+    if (act) {
+        dft8<C, INV>(v);
+#pragma unroll
+        for (int q = 0; q < 8; q++) lds[fpad(8 * tp + q)] = v[q];
+    }
+    __syncthreads();
+    if constexpr (M > 1) {
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int b = tp + i * TPR;
+                if (PER * TPR == 64 || b < 64) {
+                    C xs[M];
+                    TwPow<INV> t(tw, 8 * (b & 7));
+#pragma unroll
+                    for (int j = 0; j < M; j++) {
+                        const C a = lds[fpad(b + 64 * j)];
+                        xs[j] = j ? t.mul(a) : a;
+                    }
+                    dftR<C, INV, M>(xs);
+#pragma unroll
+                    for (int q = 0; q < M; q++) lds[fpad(b + 64 * q)] = xs[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (act) {
+        TwPow<INV> t(tw, tp);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const C a = lds[fpad(8 * j + w + 64 * u)];
+            v[j] = j ? t.mul(a) : a;
+        }
+        dft8<C, INV>(v);
+    }
+    if (END) __syncthreads();
+}
+
+// Forward transform (fft_pow), a pointwise operation op(v) on the
+// natural distribution, inverse transform: the sequence of the column pass.
+// The inverse transform's first pass writes each thread's outputs into the
+// slots that thread read in the forward transform's last pass, so no
+// barrier separates the two transforms.  Inverse slots (tp = 8u + w):
+// pass 1 writes 8 q + w + 64 u; middle butterfly b reads/writes
+// 8 (b & 7) + (b >> 3) + 64 j; pass 3 reads 8 w + j + 64 u.  Four barriers
+// for the pair; none at the end: a caller that touches `lds` afterwards
+// must synchronise first.
 template <typename C, int N, typename Op>
 __device__ __forceinline__ void fft_fwd_op_inv(C (&v)[8], C* lds, const double2* __restrict__ tw, int tp, Op op) {
     constexpr int TPR = FftShape<N>::TPR;
     constexpr int M = FftShape<N>::M;
     constexpr int PER = (64 + TPR - 1) / TPR;
     const int w = tp & 7, u = tp >> 3;
-    // forward
-    dft8<C, false>(v);
-#pragma unroll
-    for (int q = 0; q < 8; q++) lds[fpad(8 * tp + q)] = v[q];
-    __syncthreads();
-    if constexpr (M > 1) {
-#pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const int b = tp + i * TPR;
-            if (PER * TPR == 64 || b < 64) {
-                C xs[M];
-                TwPow<false> t(tw, 8 * (b & 7));
-#pragma unroll
-                for (int j = 0; j < M; j++) {
-                    const C a = lds[fpad(b + 64 * j)];
-                    xs[j] = j ? t.mul(a) : a;
-                }
-                dftR<C, false, M>(xs);
-#pragma unroll
-                for (int q = 0; q < M; q++) lds[fpad(b + 64 * q)] = xs[q];
-            }
-        }
-        __syncthreads();
-    }
-    {
-        TwPow<false> t(tw, tp);
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const C a = lds[fpad(8 * j + w + 64 * u)];
-            v[j] = j ? t.mul(a) : a;
-        }
-        dft8<C, false>(v);
-    }
+    fft_pow<C, N, false, false>(v, lds, tw, tp);
     op(v);
-    // inverse
     dft8<C, true>(v);
 #pragma unroll
     for (int q = 0; q < 8; q++) lds[fpad(8 * q + w + 64 * u)] = v[q];
@@ -341,15 +358,13 @@ __device__ __forceinline__ void fft_fwd_op_inv(C (&v)[8], C* lds, const double2*
         }
         __syncthreads();
     }
-    {
-        TwPow<true> t(tw, tp);
+    TwPow<true> t(tw, tp);
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const C a = lds[fpad(8 * w + j + 64 * u)];
-            v[j] = j ? t.mul(a) : a;
-        }
-        dft8<C, true>(v);
+    for (int j = 0; j < 8; j++) {
+        const C a = lds[fpad(8 * w + j + 64 * u)];
+        v[j] = j ? t.mul(a) : a;
     }
+    dft8<C, true>(v);
 }
 
 }  // namespace shg

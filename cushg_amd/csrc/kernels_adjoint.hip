@@ -79,7 +79,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row(AdjArgs a, const double2*
         }
         C* buf = lds + (act ? gi : 0) * LD;
         if (a.first || !a.fwd_spatial)
-            fft_dist<C, NX, true>(v, buf, tw, tp, act && (f < 2 ? a.first != 0 : !a.fwd_spatial));
+            fft_pow<C, NX, true, true>(v, buf, tw, tp, act && (f < 2 ? a.first != 0 : !a.fwd_spatial));
         if (act) {
 #pragma unroll
             for (int q = 0; q < 8; q++) buf[fpad(tp + q * TPR)] = v[q];
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row(AdjArgs a, const double2*
         }
         if (a.second) {
             __syncthreads();  // every group has read its buffer before the transform overwrites it
-            fft_dist<C, NX, false>(v, buf, tw, tp, act);
+            fft_pow<C, NX, false, false>(v, buf, tw, tp, act);
         }
         if (act) {
             S* dst = f == 0 ? LP : LS;
@@ -162,7 +162,15 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_adj_col(AdjArgs a, const doub
     const MixedInst& I = *a.I;
     const int nx = I.nx;
     const int c = threadIdx.x, tp = threadIdx.y;
-    const int x = blockIdx.x * COLS + c;
+    // the K column groups sharing a 128-byte row segment on one XCD (ids 8
+    // apart share an XCD and its L2), as in the forward column pass
+    constexpr int K = COLS * 8 >= 128 ? 1 : 128 / (COLS * 8);
+    int bx = blockIdx.x;
+    if (K > 1 && gridDim.x % (8 * K) == 0) {
+        const int lo = bx & 7, hi = (bx >> 3) % K, chunk = bx / (8 * K);
+        bx = chunk * 8 * K + lo * K + hi;
+    }
+    const int x = bx * COLS + c;
     const bool sh = blockIdx.y != 0;
     S* L = static_cast<S*>(sh ? a.lamS : a.lamP);
     const S* F = static_cast<const S*>(sh ? a.fwdS : a.fwdP);
@@ -175,23 +183,24 @@ __global__ __launch_bounds__(NY / 8 * COLS) void k_adj_col(AdjArgs a, const doub
         v[q] = from_store<C>(L[(size_t)(tp + q * TPC) * nx + x]);
         w[q] = from_store<C>(F[(size_t)(tp + q * TPC) * nx + x]);
     }
-    fft_dist<C, NY, false>(v, my, tw, tp);
-    fft_dist<C, NY, false>(w, my, tw, tp);
+    // Fy w, then Fy v -> gradient term and * conj(Prop) -> IFy
+    fft_pow<C, NY, false, true>(w, my, tw, tp);  // (its end barrier: v's transforms write next)
     const int ic = (x + nx / 2) % nx;
     const double fx = (ic - 0.5 * nx) * a.dfx;
     const double exc = ex[x].x, exs = -ex[x].y;  // conj(ex)
     double gk = 0.0;
+    fft_fwd_op_inv<C, NY>(v, my, tw, tp, [&](C (&b)[8]) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int ky = tp + q * TPC;
-        const int jc = (ky + NY / 2) % NY;
-        const double fy = (jc - 0.5 * NY) * a.dfy;
-        // Re(conj(v) i w) = v.x * (-w.y) + v.y * w.x
-        gk += (-v[q].x * w[q].y + v[q].y * w[q].x) * (fx * fx + fy * fy);
-        const double2 pr = cmul(mk<double2>(exc, exs), conjc(ey[ky]));
-        v[q] = cmul(v[q], pr);
-    }
-    fft_dist<C, NY, true>(v, my, tw, tp);
+        for (int q = 0; q < 8; q++) {
+            const int ky = tp + q * TPC;
+            const int jc = (ky + NY / 2) % NY;
+            const double fy = (jc - 0.5 * NY) * a.dfy;
+            // Re(conj(v) i w) = v.x * (-w.y) + v.y * w.x
+            gk += (-b[q].x * w[q].y + b[q].y * w[q].x) * (fx * fx + fy * fy);
+            const double2 pr = cmul(mk<double2>(exc, exs), conjc(ey[ky]));
+            b[q] = cmul(b[q], pr);
+        }
+    });
     const uint32_t key = store_key(5, sh ? 1 : 0, a.s);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -327,18 +336,22 @@ hipError_t launch_adj_row(bool f64, int nx, int ny, hipStream_t st, const AdjArg
     return f64 ? adj_row_n<double2>(nx, grid, st, a, tw) : adj_row_n<float2>(nx, grid, st, a, tw);
 }
 
-// two columns per workgroup: the adjoint runs one instance, so the grid
-// (nx / 2 x 2 fields) is what spreads it over the CUs
-constexpr int kAdjCols = 2;
+// columns per workgroup: the adjoint runs one instance, so the grid
+// (nx / COLS x 2 fields) is what spreads it over the CUs; four columns
+// (32-byte row segments) where that still gives 256 workgroups, else two
+static int adj_cols(int nx) { return nx >= 512 ? 4 : 2; }
 
 template <typename S>
 static hipError_t adj_col_n(int ny, int nx, hipStream_t st, const AdjArgs& a, const double2* tw) {
-    constexpr int COLS = kAdjCols;
-    const dim3 grid(nx / COLS, 2);
+    const int cols = adj_cols(nx);
+    const dim3 grid(nx / cols, 2);
     switch (ny) {
 #define SHG_ADJCOL(N)                                                                                 \
     case N:                                                                                           \
-        hipLaunchKernelGGL((k_adj_col<N, COLS, S>), grid, dim3(COLS, N / 8), 0, st, a, tw);          \
+        if (cols == 4)                                                                                \
+            hipLaunchKernelGGL((k_adj_col<N, 4, S>), grid, dim3(4, N / 8), 0, st, a, tw);            \
+        else                                                                                          \
+            hipLaunchKernelGGL((k_adj_col<N, 2, S>), grid, dim3(2, N / 8), 0, st, a, tw);            \
         return hipGetLastError();
         SHG_ADJCOL(64)
         SHG_ADJCOL(128)
@@ -351,7 +364,7 @@ static hipError_t adj_col_n(int ny, int nx, hipStream_t st, const AdjArgs& a, co
     return hipErrorInvalidValue;
 }
 
-int adj_col_workgroups(int nx) { return nx / kAdjCols * 2; }
+int adj_col_workgroups(int nx) { return nx / adj_cols(nx) * 2; }
 
 hipError_t launch_adj_col(bool f64, int nx, int ny, hipStream_t st, const AdjArgs& a, const double2* tw) {
     return f64 ? adj_col_n<double2>(ny, nx, st, a, tw) : adj_col_n<float2>(ny, nx, st, a, tw);

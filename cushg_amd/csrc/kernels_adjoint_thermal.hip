@@ -100,7 +100,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal 
         }
         C* buf = lds + (act ? gi : 0) * LD;
         if (a.first || !a.fwd_spatial)
-            fft_dist<C, NX, true>(v, buf, tw, tp, act && (f < 2 ? a.first != 0 : !a.fwd_spatial));
+            fft_pow<C, NX, true, true>(v, buf, tw, tp, act && (f < 2 ? a.first != 0 : !a.fwd_spatial));
         if (act) {
 #pragma unroll
             for (int q = 0; q < 8; q++) buf[fpad(tp + q * TPR)] = v[q];
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal 
         }
         if (a.second) {
             __syncthreads();  // every group has read its buffer before the transform overwrites it
-            fft_dist<C, NX, false>(v, buf, tw, tp, act);
+            fft_pow<C, NX, false, false>(v, buf, tw, tp, act);
         }
         if (act) {
             S* dst = f == 0 ? LP : LS;
