@@ -52,9 +52,10 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], double* part, double
 //   [0..5]  call R1_s:     dL/dphi_0..2, dL/dkp, dL/dks, dL/dbfac
 //   [6..11] call R2_{s-1}: the same
 template <int NX, int ROWS, typename S>
-__global__ __launch_bounds__(NX * ROWS) void k_adj_row(AdjArgs a, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row(AdjArgs a, const double2* __restrict__ tw) {
     using C = double2;
     constexpr int TPR = NX / 8;
+    constexpr int NT = NX * ROWS / 2;  // threads: the 4 ROWS transform groups, two points each in the pointwise work
     constexpr int LD = FftShape<NX>::LDS;
     __shared__ C lds[4 * ROWS * LD];
     __shared__ double red[16 * 12];
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row(AdjArgs a, const double2*
     const size_t row0 = (size_t)blockIdx.x * ROWS * NX;
     // 1. the four rows per image row into LDS, spatial
     {
-        const bool act = gi < 4 * ROWS;
+        const bool act = true;  // NT = 4 ROWS TPR: every thread is in a transform group
         const int f = gi & 3;
         const S* src = f == 0 ? LP : f == 1 ? LS : f == 2 ? fP : fS;
         const size_t base = row0 + (size_t)(gi >> 2) * NX + tp;
@@ -100,9 +101,10 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row(AdjArgs a, const double2*
         }
     }
     double acc1[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
-    {
-        C* rb = lds + (t / NX) * 4 * LD;
-        const int xi = fpad(t % NX);
+#pragma unroll 1
+    for (int pt = t; pt < NX * ROWS; pt += NT) {
+        C* rb = lds + (pt / NX) * 4 * LD;
+        const int xi = fpad(pt % NX);
         const C Yp = rb[2 * LD + xi], Ys = rb[3 * LD + xi];
         C Xp = Yp, Xs = Ys;
         if (!a.fwd_spatial) rk4_point(Xp, Xs, e2, I.kp, I.ks, I.bfac, k);  // X_s = R2_{s-1}(Y_{s-1})
@@ -305,7 +307,7 @@ static hipError_t adj_row_n(int nx, dim3 grid, hipStream_t st, const AdjArgs& a,
     switch (nx) {
 #define SHG_ADJROW(N)                                                                                 \
     case N:                                                                                           \
-        hipLaunchKernelGGL((k_adj_row<N, adj_rows<N>(), S>), grid, dim3(N * adj_rows<N>()), 0, st, a, tw);      \
+        hipLaunchKernelGGL((k_adj_row<N, adj_rows<N>(), S>), grid, dim3(N * adj_rows<N>() / 2), 0, st, a, tw);  \
         return hipGetLastError();
         SHG_ADJROW(64)
         SHG_ADJROW(128)

@@ -66,16 +66,19 @@ __device__ __forceinline__ ThCall th_call(const MixedInst& I, size_t v, int sl, 
 }
 
 // Adjoint THERMAL row pass at boundary s: ROWS rows per workgroup of
-// NX * ROWS threads, grid (ny / ROWS); the layout of k_adj_row (transforms on
-// groups of NX/8 threads, the pointwise reverse work one point per thread).
+// NX * ROWS / 2 threads, grid (ny / ROWS): the 4 ROWS transforms (lam_P,
+// lam_S, Y_P, Y_S of each row) on groups of NX/8 threads, so that every
+// wave takes part in them, and the pointwise reverse work two points per
+// thread.
 //   first       apply R1_s^T (lam is in (kx, y) and is IFx'd first)
 //   second      apply the heat-source gradient of q_{s-1} and R2_{s-1}^T,
 //               and Fx the result
 //   fwd_spatial fwd is the spatial initial state X_0 (s == 0)
 template <int NX, int ROWS, typename S>
-__global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal th, const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row_th(AdjArgs a, AdjThermal th, const double2* __restrict__ tw) {
     using C = double2;
     constexpr int TPR = NX / 8;
+    constexpr int NT = NX * ROWS / 2;  // threads: the 4 ROWS transform groups, two points each in the pointwise work
     constexpr int LD = FftShape<NX>::LDS;
     __shared__ C lds[4 * ROWS * LD];
     __shared__ double red[16 * 2];
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal 
     const size_t row0 = (size_t)blockIdx.x * ROWS * NX;
     // 1. the four rows per image row into LDS, spatial
     {
-        const bool act = gi < 4 * ROWS;
+        const bool act = true;  // NT = 4 ROWS TPR: every thread is in a transform group
         const int f = gi & 3;
         const S* src = f == 0 ? LP : f == 1 ? LS : f == 2 ? fP : fS;
         const size_t base = row0 + (size_t)(gi >> 2) * NX + tp;
@@ -111,8 +114,9 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal 
     const RkConst<double> k{I.c1, I.c2, I.c3, I.w, I.ap_h, I.as_h};
     const int s = a.s, nz = I.nz, nzh = I.nzh;
     double pol1 = 0.0, pol2 = 0.0;
-    {
-        const int r = t / NX, x = t % NX;
+#pragma unroll 1
+    for (int pt = t; pt < NX * ROWS; pt += NT) {
+        const int r = pt / NX, x = pt % NX;
         const int y = blockIdx.x * ROWS + r;
         C* rb = lds + r * 4 * LD;
         const int xi = fpad(x);
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal 
             gC0 = cs;
             gnp0 = -acc1[3] * k1.kp / k1.np;
             gns0 = -acc1[4] * k1.ks / k1.ns + acc1[5] * I.bf0;
-            pol1 = acc1[0] + acc1[1] + acc1[2];
+            pol1 += acc1[0] + acc1[1] + acc1[2];
         }
         if (th.seed != 0.0 && s == nz - 1) ls = axpy(th.seed, Xs, ls);
         if (c2) {
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(NX * ROWS) void k_adj_row_th(AdjArgs a, AdjThermal 
             gC1 += cs;
             gnp1 += -acc2[3] * k2.kp / k2.np;
             gns1 += -acc2[4] * k2.ks / k2.ns + acc2[5] * I.bf0;
-            pol2 = acc2[0] + acc2[1] + acc2[2];
+            pol2 += acc2[0] + acc2[1] + acc2[2];
         }
         // per-voxel accumulation (each point of a launch belongs to one thread)
         if (c1 && c2 && h0 == h1) {
@@ -276,7 +280,7 @@ static hipError_t adj_row_th_n(int nx, dim3 grid, hipStream_t st, const AdjArgs&
     switch (nx) {
 #define SHG_ADJROWTH(N)                                                                                   \
     case N:                                                                                               \
-        hipLaunchKernelGGL((k_adj_row_th<N, adj_rows_th<N>(), S>), grid, dim3(N * adj_rows_th<N>()), 0, st, a, t, \
+        hipLaunchKernelGGL((k_adj_row_th<N, adj_rows_th<N>(), S>), grid, dim3(N * adj_rows_th<N>() / 2), 0, st, a, t, \
                            tw);                                                                          \
         return hipGetLastError();
         SHG_ADJROWTH(64)
