@@ -188,7 +188,7 @@ def default_params() -> ShgParams:
     p.T_inf_C = 25.0
     p.T_peltier_C = 55.66
     p.thermal = 0
-    p.precision = PREC_FP32_MIXED
+    p.precision = PREC_FP32_STORAGE
     p.heat_solver = HEAT_JACOBI_REFERENCE
     p.max_outer = 20
     p.check_every = 5000

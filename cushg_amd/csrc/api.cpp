@@ -102,7 +102,7 @@ void shg_params_default(shg_params* p) {
     p->T_inf_C = 25.0;  // src/cuSHG.cu:52
     p->T_peltier_C = 55.66;
     p->thermal = 0;
-    p->precision = SHG_PREC_FP32_MIXED;
+    p->precision = SHG_PREC_FP32_STORAGE;  // the FP32-field form within the 1e-5 eta bar on every measured case
     p->heat_solver = SHG_HEAT_JACOBI_REFERENCE;
     p->max_outer = 20;       // src/headers/Solver.h:325
     p->check_every = 5000;   // src/headers/Solver.h:342

@@ -138,7 +138,7 @@ typedef struct shg_params {
 
     /* THERMAL (src/headers/Solver.h:321-373) */
     int32_t thermal;         /* 0/1: the reference's -DTHERMAL */
-    int32_t precision;       /* SHG_PREC_* */
+    int32_t precision;       /* SHG_PREC_*; default SHG_PREC_FP32_STORAGE */
     int32_t heat_solver;     /* SHG_HEAT_* */
     int32_t max_outer;       /* outer thermo-optic passes; reference 20 */
     int32_t check_every;     /* Jacobi check period; reference 5000 */
