@@ -97,6 +97,10 @@ def adjoint_cost(h) -> dict:
 DEFAULT_FORM = {"sweep64": "storage"}
 
 
+# handles (one host thread each) that take the ascent step's gradients side by side
+GRAD_HANDLES = 4
+
+
 def adjoint_thermal_cost(h, batch: int) -> dict:
     """THERMAL gradient cost at the full coupled case: one MgO:PPLN
     512x512x4000 instance with a 512x512x400 heat grid (configs.ppln512_thermal),
@@ -105,7 +109,7 @@ def adjoint_thermal_cost(h, batch: int) -> dict:
     two calls; then, with batch > 0, one eta-maximisation step
     (cushg_amd.optimize.ascent_step) over that many seeded (waist, focus,
     oven) triples of the same case."""
-    from cushg_amd import abi, configs, optimize
+    from cushg_amd import abi, configs, optimize, shg
     p = configs.ppln512_thermal()
     p.precision = abi.PREC_FP32_STORAGE
     h.run(p, fields=False)
@@ -126,7 +130,13 @@ def adjoint_thermal_cost(h, batch: int) -> dict:
         ps = configs.ppln512_thermal_batch(batch)
         for q in ps:
             q.precision = abi.PREC_FP32_STORAGE
-        st = optimize.ascent_step(h, ps)
+        st1 = optimize.ascent_step(h, ps)
+        extra = [shg.Handle(0) for _ in range(GRAD_HANDLES - 1)]
+        try:
+            st = optimize.ascent_step(h, ps, extra_handles=extra)
+        finally:
+            for e in extra:
+                e.close()
         t = time.perf_counter()
         for q in ps:
             h.run(q, fields=False)
@@ -136,15 +146,19 @@ def adjoint_thermal_cost(h, batch: int) -> dict:
         rc, rb, _ = h.run_batch(ps)
         tb = time.perf_counter() - t
         gains = [s.eta1 / s.eta0 - 1 for s in st.steps]
-        out["ascent_step"] = {"instances": batch, "forward_plus_adjoint_s": round(st.grad_wall_s, 2),
-                              "forward_s": round(ts, 2), "ratio": round(st.grad_wall_s / ts, 3),
+        out["ascent_step"] = {"instances": batch, "forward_plus_adjoint_s": round(st1.grad_wall_s, 2),
+                              "forward_s": round(ts, 2), "ratio": round(st1.grad_wall_s / ts, 3),
                               "ratio_basis": "the instances' forward + adjoint calls over their forward calls, "
-                                             "both one instance at a time",
+                                             "both one instance at a time on one handle",
+                              "step_s_one_handle": round(st1.wall_s, 2),
+                              "gradient_handles": GRAD_HANDLES,
+                              "forward_plus_adjoint_s_handles": round(st.grad_wall_s, 2),
                               "step_s": round(st.wall_s, 2), "batched_forward_s": round(tb, 2),
                               "accepted": sum(s.accepted for s in st.steps),
                               "eta_gain_rel": {"min": min(gains), "max": max(gains)}}
-        print(f"bench: ascent step over {batch}: {st.wall_s:.1f} s (forward + adjoint {st.grad_wall_s:.1f} s, "
-              f"forward {ts:.1f} s, ratio {st.grad_wall_s / ts:.2f})", file=sys.stderr, flush=True)
+        print(f"bench: ascent step over {batch}: {st1.wall_s:.1f} s on one handle (forward + adjoint "
+              f"{st1.grad_wall_s:.1f} s, forward {ts:.1f} s, ratio {st1.grad_wall_s / ts:.2f}); {st.wall_s:.1f} s with "
+              f"the gradients on {GRAD_HANDLES} handles ({st.grad_wall_s:.1f} s)", file=sys.stderr, flush=True)
     return out
 
 

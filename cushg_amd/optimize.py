@@ -15,7 +15,9 @@ the new point raises eta and halved (at most `backtrack` times) otherwise.
 from __future__ import annotations
 
 import math
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -61,17 +63,44 @@ def _moved(p: ShgParams, d) -> ShgParams:
     return q
 
 
-def ascent_step(h: shg.Handle, ps, radii=RADII, backtrack: int = 3, checkpoint_every: int = 0) -> BatchStep:
+def gradients(hs, ps, checkpoint_every: int = 0):
+    """(eta, d eta / d(waist, focus, T)) of every triple in `ps` by
+    shg_adjoint.  With several handles (on one device, each used by one host
+    thread) the instances are taken in turn by one thread per handle, so that
+    the single-instance adjoints, which do not fill the GPU alone, run side
+    by side on the handles' streams."""
+    out = [None] * len(ps)
+    nxt = iter(range(len(ps)))
+    lock = threading.Lock()
+
+    def work(h):
+        while True:
+            with lock:
+                i = next(nxt, None)
+            if i is None:
+                return
+            res, g = h.adjoint(ps[i], checkpoint_every)
+            out[i] = (res.eta, np.array(g[:3], dtype=np.float64))
+
+    if len(hs) == 1:
+        work(hs[0])
+    else:
+        with ThreadPoolExecutor(len(hs)) as ex:
+            for f in [ex.submit(work, h) for h in hs]:
+                f.result()
+    return out
+
+
+def ascent_step(h: shg.Handle, ps, radii=RADII, backtrack: int = 3, checkpoint_every: int = 0,
+                extra_handles=()) -> BatchStep:
     """One eta-maximisation step for every triple in `ps` (ShgParams; T_C
     and T_peltier_C move together).  Gradients by shg_adjoint, one instance
-    after another; the trial points of all instances run as one batch."""
+    at a time per handle (`h` and `extra_handles`, see gradients()); the
+    trial points of all instances run as one batch on `h`."""
     t0 = time.perf_counter()
     r = np.asarray(radii, dtype=np.float64)
     out = BatchStep()
-    evals = []
-    for p in ps:
-        res, g = h.adjoint(p, checkpoint_every)
-        evals.append((res.eta, np.array(g[:3], dtype=np.float64)))
+    evals = gradients([h, *extra_handles], ps, checkpoint_every)
     out.grad_wall_s = time.perf_counter() - t0
     scale = [1.0] * len(ps)
     pending = list(range(len(ps)))

@@ -56,3 +56,20 @@ def test_small_step_matches_directional_derivative(handle):
         assert lin > 0
         assert abs((s.eta1 - s.eta0) - lin) <= 0.005 * lin
         assert np.all(np.abs(d) <= np.array(radii) * (1 + 1e-12))
+
+
+def test_gradients_on_several_handles_equal_one_handle(handle):
+    """Gradients taken side by side on three handles (one host thread each)
+    equal those taken one after another on one handle."""
+    ps = triples(5)
+    print("case: gradients of 5 THERMAL PPLN 64x64x24 triples, 1 handle vs 3", flush=True)
+    one = optimize.gradients([handle], ps)
+    extra = [shg.Handle(0), shg.Handle(0)]
+    try:
+        three = optimize.gradients([handle, *extra], ps)
+    finally:
+        for h in extra:
+            h.close()
+    for (e1, g1), (e3, g3) in zip(one, three):
+        assert e1 == e3
+        assert np.allclose(g1, g3, rtol=1e-10, atol=0.0)
