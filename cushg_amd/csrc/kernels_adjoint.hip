@@ -42,12 +42,12 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], double* part, double
     }
 }
 
-// Adjoint row pass at boundary s: ROWS rows per workgroup of NX * ROWS
+// Adjoint row pass at boundary s: ROWS rows per workgroup of NX * ROWS / 2
 // threads, grid (ny / ROWS).  The transforms run on groups of NX/8 threads
 // holding 8 points each (group g: row g / 4, field g % 4 = lam_P, lam_S,
-// Y_P, Y_S; the groups beyond 4 ROWS only join the barriers); the pointwise
-// reverse RK4 work, which dominates, runs one point per thread from LDS so
-// that a single 256 x 256 instance still spreads over every SIMD.
+// Y_P, Y_S: every thread is in one); the pointwise reverse RK4 work, which
+// dominates, runs two points per thread from LDS, so that a single
+// 256 x 256 instance still spreads over every SIMD.
 // Partial sums per workgroup (12):
 //   [0..5]  call R1_s:     dL/dphi_0..2, dL/dkp, dL/dks, dL/dbfac
 //   [6..11] call R2_{s-1}: the same
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row(AdjArgs a, const doub
         }
     }
     __syncthreads();
-    // 2. one point per thread
+    // 2. the pointwise reverse work, two points per thread
     const RkConst<double> k{I.c1, I.c2, I.c3, I.w, I.ap_h, I.as_h};
     const int s = a.s;
     C e1[3], e2[3];

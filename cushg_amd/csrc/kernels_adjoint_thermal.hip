@@ -110,7 +110,7 @@ __global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row_th(AdjArgs a, AdjTher
         }
     }
     __syncthreads();
-    // 2. one point per thread
+    // 2. the pointwise reverse work, two points per thread
     const RkConst<double> k{I.c1, I.c2, I.c3, I.w, I.ap_h, I.as_h};
     const int s = a.s, nz = I.nz, nzh = I.nzh;
     double pol1 = 0.0, pol2 = 0.0;
