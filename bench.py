@@ -338,6 +338,9 @@ def main():
                     help="skip the adjoint cost lines (256x256, 2000 slices; and THERMAL 512x512x4000)")
     ap.add_argument("--ascent-batch", type=int, default=16,
                     help="instances of the THERMAL eta-maximisation step timed in the adjoint line (0: none)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the THERMAL adjoint cost of one 512x512x4000 instance (profiling runs): prints its "
+                         "JSON object and exits")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC counter calibration)")
     args = ap.parse_args()
@@ -363,6 +366,10 @@ def main():
 
     from cushg_amd import abi, shg
 
+    if args.adjoint_only:
+        with shg.Handle(0) as h:
+            print(json.dumps(adjoint_thermal_cost(h, 0)))
+        return
     ps, desc = workload(args.workload)
     forms = {"split": abi.PREC_FP32_SPLIT, "storage": abi.PREC_FP32_STORAGE, "mixed": abi.PREC_FP32_MIXED}
     names = {v: k for k, v in forms.items()}
