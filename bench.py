@@ -22,6 +22,8 @@ The JSON line also carries:
                   records (hipExtLaunchKernel) on the library's stream;
                   `traffic` = HBM bytes per launch from the committed
                   rocprofv3 counter summary (profiles/pmc_traffic.json).
+  roofline_other_pass  the same for the other fused pass (row or column)
+                  of the same profiled run.
   fp64_plain      the plain FP64 GPU form (one instance at a time) over
                   every instance: its throughput, and
   eta_rel_err_vs_fp64  the eta error of the timed form against it (and
@@ -210,14 +212,27 @@ def kernel_roofline(h, ps):
     """Dominant kernel's achieved algorithmic bandwidth: its algorithmic
     bytes summed over every launch of a profiled run (each launch covers the
     instances still marching in it), divided by the kernel's total time from
-    HIP events recorded on the library's stream."""
+    HIP events recorded on the library's stream.  Also returns the same
+    figures for the other fused pass of the run (row or column) and the
+    per-kernel times."""
     h.set_profiling(True)
     res = run_once(h, ps)
     kt = h.kernel_times()
     h.set_profiling(False)
     if not kt:
-        return None, kt
+        return None, None, kt
     name, (ms, n) = max(kt.items(), key=lambda kv: kv[1][0])
+    other = None
+    fused = [k for k in kt if k.split("_")[0] in BYTES_PER_PT and k != name]
+    if fused:
+        o = max(fused, key=lambda k: kt[k][0])
+        other = pass_roofline(o, kt[o][0], kt[o][1], ps, res)
+    return pass_roofline(name, ms, n, ps, res), other, kt
+
+
+def pass_roofline(name, ms, n, ps, res):
+    """Roofline object of one fused-pass kernel (profiling name, total ms,
+    launches) over the profiled run's instances and results."""
     from cushg_amd import abi
     parts = name.split("_")
     base, size = parts[0], int(parts[1])
@@ -229,7 +244,7 @@ def kernel_roofline(h, ps):
                  "colpass": (abi.PREC_FP32_STORAGE, abi.PREC_FP32_SPLIT)}.get(base, ())
     if base not in BYTES_PER_PT:
         return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": None, "traffic": None}, kt
+                "frac": None, "traffic": None}
     total = 0.0
     # the same launches priced at the minimum of the two-pass scheme: 32 B
     # per point and pass for the fields, plus 24 B per point and z-step for
@@ -255,6 +270,9 @@ def kernel_roofline(h, ps):
     tr = committed_traffic(name)
     alg = total / n
     traffic = round(alg * tr["traffic_over_algorithmic"], 1) if tr else None
+    layout = ("the layout's: fields 32 B/pt; THERMAL (n_p, n_s) float2, dk-base and dk doubles read per launch and Q "
+              "read-modify-written (about 66 B/pt per launch)") if base == "rowpass" and thermal else \
+        "the layout's: fields 32 B/pt (both fields read and written once)"
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": ("not measured in this run: this run's algorithmic bytes per launch x the measured "
@@ -262,14 +280,13 @@ def kernel_roofline(h, ps):
                                "request counts and WRITE_SIZE, on a 200-step, 20-slab cut of the workload)")
                               if tr else None,
             "algorithmic_bytes_per_launch": round(total / n, 1),
-            "algorithmic_bytes": "the layout's: fields 32 B/pt; THERMAL (n_p, n_s) float2, dk-base and dk doubles "
-                                 "read per launch and Q read-modify-written (about 66 B/pt per launch)",
+            "algorithmic_bytes": layout,
             "achieved_at_minimum_bytes": round(ach_s, 1), "frac_at_minimum_bytes": round(ach_s / HBM_PEAK_GBS, 4),
             "minimum_bytes_per_launch": round(survey / n, 1),
             "minimum_bytes": "32 B/pt per pass, + 24 B/pt per z-step for THERMAL (T 4, Q 4, FP64 dk integral 16)",
             "avg_launch_us": round(ms / n * 1e3, 3),
             "launches": n, "timer": "HIP events recorded by each launch's own dispatch (hipExtLaunchKernel) on the "
-                                    "library stream"}, kt
+                                    "library stream"}
 
 
 def cpu_baseline(ps, budget_s: float):
@@ -405,7 +422,7 @@ def main():
     value = work * args.steps / dt
     print(f"bench: timed {dt:.2f} s", file=sys.stderr, flush=True)
 
-    roof, kt = (None, {}) if args.no_roofline else kernel_roofline(h, ps)
+    roof, roof2, kt = (None, None, {}) if args.no_roofline else kernel_roofline(h, ps)
 
     extra = {}
     alt = None
@@ -495,6 +512,8 @@ def main():
                    "outer_iters": [r.outer_iters for r in res]},
         "roofline": roof,
     }
+    if roof2:
+        line["roofline_other_pass"] = roof2
     line.update(extra)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ps, args.cpu_budget)
