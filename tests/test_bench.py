@@ -60,3 +60,36 @@ def test_rowpass_thermal_bytes_per_point_of_the_bench_workload():
     assert math.isclose(per, 65.99, abs_tol=0.01)
     per = bench.rowpass_thermal_march_bytes(_P(512, 200, 20)) / (512 * 512 * 201)
     assert math.isclose(per, 65.83, abs_tol=0.01)
+
+
+class _Q:
+    def __init__(self, n, nz, precision, thermal=0, nzh=0):
+        self.nx = self.ny = n
+        self.nz = nz
+        self.nz_heat = nzh
+        self.precision = precision
+        self.thermal = thermal
+
+
+class _R:
+    def __init__(self, steps, outer):
+        self.steps_done = steps
+        self.outer_iters = outer
+
+
+def test_column_pass_roofline_counts_every_launch():
+    """pass_roofline for the FP64-arithmetic column pass of the split form:
+    32 B per grid point and z-step over the instances of its size and form,
+    divided by the kernel's time; other sizes and forms are not counted."""
+    from cushg_amd import abi
+    ps = [_Q(512, 100, abi.PREC_FP32_SPLIT), _Q(512, 60, abi.PREC_FP32_STORAGE), _Q(256, 100, abi.PREC_FP32_SPLIT),
+          _Q(512, 100, abi.PREC_FP32_MIXED)]
+    rs = [_R(100, 1), _R(60, 1), _R(100, 1), _R(100, 1)]
+    ms, n = 2.0, 100
+    r = bench.pass_roofline("colpass_512_f64", ms, n, ps, rs)
+    alg = 32.0 * 512 * 512 * (100 + 60)
+    assert math.isclose(r["achieved"], round(alg / (ms * 1e-3) / 1e9, 1))
+    assert math.isclose(r["frac"], round(alg / (ms * 1e-3) / 1e9 / bench.HBM_PEAK_GBS, 4))
+    assert r["frac_at_minimum_bytes"] == r["frac"]  # the column pass moves exactly its minimum
+    assert math.isclose(r["algorithmic_bytes_per_launch"], round(alg / n, 1))
+    assert "32 B/pt" in r["algorithmic_bytes"]
