@@ -6,7 +6,7 @@
     host pass only, `python cushg_amd/build.py --sanitize`) with the driver
     tests/native/host_sanity.cpp: parameter handling, validation, the
     crystal database, the CLI's argument checks and handle creation failing
-    without a GPU.  On a GPU box the same driver with argument `gpu` also
+    without a GPU.  On a machine with a GPU the same driver with argument `gpu` also
     runs every form, THERMAL solver and boundary set, batches, the trace,
     the adjoint, plans and the CLI under the sanitizers.
 """
