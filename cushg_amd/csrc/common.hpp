@@ -90,32 +90,24 @@ __host__ __device__ __forceinline__ double delta_k(int crystal, double lp, doubl
     return 2.0 * kPi * (2.0 * np / lp - ns / ls + 1.0 / (Lambda * (1.0 + ea * d + eb * d * d)));
 }
 
-// exp(i ph) for an FP64 phase: ph / (2 pi) is reduced in FP64 to the
-// nearest quarter turn n/4 and a remainder a with |a| <= pi/4, whose sine
-// and cosine come from their Taylor series in FP32 (terms through a^9 and
-// a^10; truncation error below 2e-9 on that interval), rotated by n quarter
-// turns.  Maximum error 9e-8 per component, unbiased (checked against FP64
-// sin/cos over 2.7 million phases up to |ph| = 600).
+// exp(i ph) for an FP64 phase: ph / (2 pi) is reduced in FP64 to a
+// fraction of a turn r in [-1/2, 1/2], rounded once to FP32, and the
+// hardware sine and cosine of that fraction (v_sin_f32 / v_cos_f32 take
+// their argument in turns) give the components.  Maximum error 1.9e-7 per
+// component; the magnitude is low by 3e-8 on average and the phase
+// unbiased (checked against FP64 sin/cos over 4.2 million phases up to
+// |ph| = 600), which scales a coupling term by 1 - 3e-8: far below the FP32
+// fields' own rounding.
 __device__ __forceinline__ float2 expi_fast(double ph) {
     const double t = ph * (0.5 / kPi);
-    const double n = rint(4.0 * t);
-    const float a = (float)((t - 0.25 * n) * (2.0 * kPi));
-    const float x2 = a * a;
-    const float s = a * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f +
-                                                                               x2 * (1.0f / 362880.0f)))));
-    const float c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f +
-                                                                   x2 * (1.0f / 40320.0f +
-                                                                         x2 * (-1.0f / 3628800.0f)))));
-    const int q = (int)(n - 4.0 * floor(0.25 * n));  // quarter turns mod 4
-    const float re = (q & 1) ? s : c;
-    const float im = (q & 1) ? c : s;
-    // q = 0: (c, s); 1: (-s, c); 2: (-c, -s); 3: (s, -c)
-    return make_float2((q == 1 || q == 2) ? -re : re, (q >= 2) ? -im : im);
+    const float r = (float)(t - rint(t));
+    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
 }
 
-// FP64 counterpart of expi_fast: the same quarter-turn reduction with the
-// Taylor series of sin and cos through a^15 and a^16 in FP64 (truncation
-// error below 5e-17 for |a| <= pi/4).
+// exp(i ph) in FP64: ph / (2 pi) reduced to the nearest quarter turn n/4
+// and a remainder a with |a| <= pi/4, the Taylor series of sin and cos
+// through a^15 and a^16 in FP64 (truncation error below 5e-17), rotated by
+// n quarter turns.
 __device__ __forceinline__ double2 expi_fast64(double ph) {
     const double t = ph * (0.5 / kPi);
     const double n = rint(4.0 * t);
