@@ -31,6 +31,23 @@ template <> struct cpx_t<float> { using type = float2; };
 template <> struct cpx_t<double> { using type = double2; };
 template <typename T> using cpx = typename cpx_t<T>::type;
 
+// Load / store through the global address space (global_load / global_store
+// instead of the flat forms a generic pointer gets).  Global loads complete
+// in issue order on the vector-memory counter alone, so a load issued early
+// stays in flight while the wait for an earlier one is satisfied; a flat
+// access also counts on the LDS counter, whose waits are all-or-nothing.
+template <typename T> __device__ __forceinline__ T ld_global(const T* p) {
+    if constexpr (std::is_arithmetic_v<T>) {
+        return *(const __attribute__((address_space(1))) T*)p;
+    } else {  // an 8-byte HIP vector type: one 64-bit load
+        static_assert(sizeof(T) == 8, "ld_global: scalar or 8-byte vector");
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) unsigned long long*)p);
+    }
+}
+template <typename T> __device__ __forceinline__ void st_global(T* p, T v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+
 template <typename C> __host__ __device__ __forceinline__ C mk(decltype(C{}.x) a, decltype(C{}.x) b) {
     C c;
     c.x = a;

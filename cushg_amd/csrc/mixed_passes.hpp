@@ -233,7 +233,43 @@ template <bool THERMAL, typename S, typename C> struct NlRow {
         }
     }
 
-    __device__ __forceinline__ void point(C& p, C& s, int x, size_t i) const {
+    // THERMAL, FP32 arithmetic: the heat-grid loads of a row's next point
+    // are issued before this point's arithmetic (nl_ahead), through the
+    // global address space, so that they stay in flight while it runs (the
+    // vector-memory counter completes them in order).  FP64 arithmetic has
+    // no registers to spare for them: each point loads its own, in place.
+    static constexpr bool kAhead = THERMAL && sizeof(C) == 8;
+    // What a point reads from the heat grid for column x: the voxel of the
+    // first call that runs, and the heat source it adds to.
+    struct Vox {
+        float2 n;       // n_p(T), n_s(T)
+        double dkb, dkv;
+        float q;        // heat source before this slice's deposit (unused on a slab's first slice)
+    };
+    // Q is read unconditionally (its value is unused when !c1 or on a
+    // slab's first slice): a load on one side of a branch would make the
+    // wait at the join wait for every load in flight.
+    __device__ __forceinline__ Vox load(int x) const {
+        Vox o{};
+        if constexpr (THERMAL) {
+            const size_t v = (c1 ? row1 : row0) + x;
+            o.n = ld_global(I.nn + v);
+            o.dkb = ld_global(I.dkbase + v);
+            o.dkv = ld_global(I.dk + v);
+            o.q = ld_global(I.Q + row1 + x);
+        }
+        return o;
+    }
+
+    // one point, its heat-grid values loaded in place
+    __device__ __forceinline__ void point(C& p, C& s, int x, size_t i) const { point_t<false>(p, s, x, i, nullptr); }
+    // one point, its heat-grid values loaded ahead by load(x)
+    __device__ __forceinline__ void point(C& p, C& s, int x, size_t i, const Vox& vx) const {
+        point_t<true>(p, s, x, i, &vx);
+    }
+
+    template <bool AHEAD>
+    __device__ __forceinline__ void point_t(C& p, C& s, int x, size_t i, const Vox* vx) const {
         if constexpr (!THERMAL) {
             if (c1) {
                 rk4_point(p, s, e1, kp, ks, bfac, k);
@@ -242,9 +278,17 @@ template <bool THERMAL, typename S, typename C> struct NlRow {
             if (c0) rk4_point(p, s, e0, kp, ks, bfac, k);
         } else {
             const size_t v = (c1 ? row1 : row0) + x;
-            float2 n = I.nn[v];  // n_p(T), n_s(T) of the voxel
-            double dkb = I.dkbase[v];
-            double dkv = I.dk[v];
+            float2 n;  // n_p(T), n_s(T) of the voxel
+            double dkb, dkv;
+            if constexpr (AHEAD) {
+                n = vx->n;
+                dkb = vx->dkb;
+                dkv = vx->dkv;
+            } else {
+                n = I.nn[v];
+                dkb = I.dkbase[v];
+                dkv = I.dk[v];
+            }
             R np = n.x, ns = n.y;
             R kpv = kp0 * recip<R>(n.x), ksv = ks0 * recip<R>(n.y), bfv = bf0 * ns;
             C e0v = C{};
@@ -264,14 +308,21 @@ template <bool THERMAL, typename S, typename C> struct NlRow {
                 const R Is = hce * ns * abs2(s);
                 const R qv = ((R)I.alpha_p * Ip + (R)I.alpha_s * Is + (R)I.beta_s * Is * Is) * qw;
                 float* Qp = I.Q + row1 + x;
-                *Qp = first ? (float)qv : (float)((R)*Qp + qv);
+                if constexpr (AHEAD) st_global(Qp, first ? (float)qv : (float)((R)vx->q + qv));
+                else *Qp = first ? (float)qv : (float)((R)*Qp + qv);
                 emit(p, s, i);
             }
             if (c0) {
                 if (c1 && h0 != h1) {  // the call falls in the next heat slab
-                    n = I.nn[row0 + x];
-                    dkb = I.dkbase[row0 + x];
-                    dkv = I.dk[row0 + x];
+                    if constexpr (AHEAD) {
+                        n = ld_global(I.nn + row0 + x);
+                        dkb = ld_global(I.dkbase + row0 + x);
+                        dkv = ld_global(I.dk + row0 + x);
+                    } else {
+                        n = I.nn[row0 + x];
+                        dkb = I.dkbase[row0 + x];
+                        dkv = I.dk[row0 + x];
+                    }
                     np = n.x;
                     ns = n.y;
                     kpv = kp0 * recip<R>(n.x);
@@ -290,10 +341,25 @@ template <bool THERMAL, typename S, typename C> struct NlRow {
     }
 };
 
-// the row's 8 points x = tp + q*TPR of one thread
+// The nonlinear work of a thread's 8 points x = tp + q*TPR of one row.
+// Loads ahead (Nl::kAhead): cur holds the first point's heat-grid values,
+// and each point's loads are issued before the previous point's arithmetic.
+template <int NX, typename Nl, typename C>
+__device__ __forceinline__ void nl_ahead(C (&p)[8], C (&s)[8], const Nl& nl, int tp, size_t base,
+                                         typename Nl::Vox cur) {
+    constexpr int TPR = NX / 8;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        typename Nl::Vox nxt{};
+        if (q < 7) nxt = nl.load(tp + (q + 1) * TPR);
+        nl.point(p[q], s[q], tp + q * TPR, base + q * TPR, cur);
+        cur = nxt;
+    }
+}
+// Loads in place.
 template <int NX, bool THERMAL, typename S, typename C>
-__device__ __forceinline__ void nl_pair(C (&p)[8], C (&s)[8], const MixedInst& I, int g, int y, int tp,
-                                        size_t base, bool trace) {
+__device__ __forceinline__ void nl_inplace(C (&p)[8], C (&s)[8], const MixedInst& I, int g, int y, int tp,
+                                           size_t base, bool trace) {
     constexpr int TPR = NX / 8;
     const NlRow<THERMAL, S, C> nl(I, g, y, trace);
 #pragma unroll
@@ -358,8 +424,20 @@ void k_rowpass(const MixedInst* __restrict__ insts, Step gs, const double2* __re
             static_cast<S*>(I.trS)[i] = to_store<S>(s[q], i, store_key(3, 1, g));
         }
     }
-    if (g > 0) fft_pair<C, NX, true>(f, my, ROWS * LD, tw, tp);
-    nl_pair<NX, THERMAL, S, C>(p, s, I, g, y, tp, base, trace);
+    using Nl = NlRow<THERMAL, S, C>;
+    // Loads ahead from 192 points up (below, their registers would cost a
+    // wave per SIMD).  The first point's loads are issued after the row
+    // IFFT: issued before it, they were waited for with the IFFT's twiddle
+    // loads (vector-memory loads complete in order), which made the pass
+    // slower (44.6 against 41.2 us per 16-instance 512x512 launch).
+    if constexpr (Nl::kAhead && NX >= 192) {
+        const Nl nl(I, g, y, trace);
+        if (g > 0) fft_pair<C, NX, true>(f, my, ROWS * LD, tw, tp);
+        nl_ahead<NX>(p, s, nl, tp, base, nl.load(tp));
+    } else {
+        if (g > 0) fft_pair<C, NX, true>(f, my, ROWS * LD, tw, tp);
+        nl_inplace<NX, THERMAL, S, C>(p, s, I, g, y, tp, base, trace);
+    }
     if (g < nz) fft_pair<C, NX, false>(f, my, ROWS * LD, tw, tp);
     const uint32_t kp = store_key(0, 0, g), ks = store_key(0, 1, g);
 #pragma unroll
