@@ -49,15 +49,24 @@ struct ThCall {
     double2 e[3];
     double kp, ks, bf, np, ns, dki, zs;
 };
-__device__ __forceinline__ ThCall th_call(const MixedInst& I, size_t v, int sl, const double (&off)[3]) {
+// the heat-grid values of voxel v that a call reads, loaded through the
+// global address space (issued together, ahead of their use)
+struct ThVox {
+    float2 n;
+    double dkb, dk;
+};
+__device__ __forceinline__ ThVox th_vox(const MixedInst& I, size_t v) {
+    return ThVox{ld_global(I.nn + v), ld_global(I.dkbase + v), ld_global(I.dk + v)};
+}
+__device__ __forceinline__ ThCall th_call(const MixedInst& I, const ThVox& x, int sl, const double (&off)[3]) {
     ThCall c;
-    const float2 n = I.nn[v];
+    const float2 n = x.n;
     c.np = n.x;
     c.ns = n.y;
     c.kp = I.kp0 * recip<double>(n.x);
     c.ks = I.ks0 * recip<double>(n.y);
     c.bf = I.bf0 * (double)n.y;
-    c.dki = I.dkbase[v] + (double)sl * I.dk[v];
+    c.dki = x.dkb + (double)sl * x.dk;
     c.zs = I.dz / ((double)sl + 1.0);
     const double pol = I.poling ? I.poling[sl] : 0.0;
 #pragma unroll
@@ -126,18 +135,22 @@ __global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row_th(AdjArgs a, AdjTher
         const size_t v0 = (size_t)h0 * plane + pix, v1 = (size_t)h1 * plane + pix;
         const double off0[3] = {I.off[0][0], I.off[0][1], I.off[0][2]};
         const double off1[3] = {I.off[1][0], I.off[1][1], I.off[1][2]};
+        // the point's heat-grid loads, all in flight before any is used (both
+        // voxels are valid rows whether or not their call runs)
+        const ThVox x1 = th_vox(I, v1), x0 = th_vox(I, v0);
+        const double w1 = c2 && th.w ? ld_global(th.w + v1) : 0.0;
         const C Yp = rb[2 * LD + xi], Ys = rb[3 * LD + xi];
         C Xp = Yp, Xs = Ys;
         ThCall k2{};
         if (c2 || !a.fwd_spatial) {
-            k2 = th_call(I, v1, s - 1, off1);
+            k2 = th_call(I, x1, s - 1, off1);
             rk4_point(Xp, Xs, k2.e, k2.kp, k2.ks, k2.bf, k);  // X_s = R2_{s-1}(Y_{s-1})
         }
         C lp = rb[xi], ls = rb[LD + xi];
         double acc1[6] = {0, 0, 0, 0, 0, 0}, acc2[6] = {0, 0, 0, 0, 0, 0};
         double gnp0 = 0, gns0 = 0, gA0 = 0, gC0 = 0, gnp1 = 0, gns1 = 0, gA1 = 0, gC1 = 0;
         if (c1) {
-            const ThCall k1 = th_call(I, v0, s, off0);
+            const ThCall k1 = th_call(I, x0, s, off0);
             rk4_adj(lp, ls, Xp, Xs, k1.e, k1.kp, k1.ks, k1.bf, k, acc1);
             double cs = 0.0;
 #pragma unroll
@@ -149,12 +162,29 @@ __global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row_th(AdjArgs a, AdjTher
             gns0 = -acc1[4] * k1.ks / k1.ns + acc1[5] * I.bf0;
             pol1 += acc1[0] + acc1[1] + acc1[2];
         }
+        // the per-voxel accumulators' current values, loaded ahead of the
+        // second call's reverse step (each point of a launch belongs to one
+        // thread, so the read-modify-write has no other writer)
+        const bool same = c1 && c2 && h0 == h1;
+        double q0[4] = {0, 0, 0, 0}, q1[4] = {0, 0, 0, 0};
+        if (c1) {
+            q0[0] = ld_global(th.gnp + v0);
+            q0[1] = ld_global(th.gns + v0);
+            q0[2] = ld_global(th.gA + v0);
+            q0[3] = ld_global(th.gC + v0);
+        }
+        if (c2 && !same) {
+            q1[0] = ld_global(th.gnp + v1);
+            q1[1] = ld_global(th.gns + v1);
+            q1[2] = ld_global(th.gA + v1);
+            q1[3] = ld_global(th.gC + v1);
+        }
         if (th.seed != 0.0 && s == nz - 1) ls = axpy(th.seed, Xs, ls);
         if (c2) {
             if (th.w) {
                 // heat source of slice s-1 at X_s: q = a_p I_p + a_s I_s + b_s I_s^2,
                 // I = hce n |A|^2, averaged over the slab's slices
-                const double wq = th.w[v1] / (double)heat_count(h1, nz, nzh);
+                const double wq = w1 / (double)heat_count(h1, nz, nzh);
                 const double ap2 = abs2(Xp), as2 = abs2(Xs);
                 const double Is = I.hce * k2.ns * as2;
                 const double dqp = I.alpha_p * I.hce * k2.np;                         // dq / d|P|^2
@@ -175,24 +205,24 @@ __global__ __launch_bounds__(NX * ROWS / 2) void k_adj_row_th(AdjArgs a, AdjTher
             gns1 += -acc2[4] * k2.ks / k2.ns + acc2[5] * I.bf0;
             pol2 += acc2[0] + acc2[1] + acc2[2];
         }
-        // per-voxel accumulation (each point of a launch belongs to one thread)
-        if (c1 && c2 && h0 == h1) {
-            th.gnp[v0] += gnp0 + gnp1;
-            th.gns[v0] += gns0 + gns1;
-            th.gA[v0] += gA0 + gA1;
-            th.gC[v0] += gC0 + gC1;
+        // per-voxel accumulation
+        if (same) {
+            st_global(th.gnp + v0, q0[0] + (gnp0 + gnp1));
+            st_global(th.gns + v0, q0[1] + (gns0 + gns1));
+            st_global(th.gA + v0, q0[2] + (gA0 + gA1));
+            st_global(th.gC + v0, q0[3] + (gC0 + gC1));
         } else {
             if (c1) {
-                th.gnp[v0] += gnp0;
-                th.gns[v0] += gns0;
-                th.gA[v0] += gA0;
-                th.gC[v0] += gC0;
+                st_global(th.gnp + v0, q0[0] + gnp0);
+                st_global(th.gns + v0, q0[1] + gns0);
+                st_global(th.gA + v0, q0[2] + gA0);
+                st_global(th.gC + v0, q0[3] + gC0);
             }
             if (c2) {
-                th.gnp[v1] += gnp1;
-                th.gns[v1] += gns1;
-                th.gA[v1] += gA1;
-                th.gC[v1] += gC1;
+                st_global(th.gnp + v1, q1[0] + gnp1);
+                st_global(th.gns + v1, q1[1] + gns1);
+                st_global(th.gA + v1, q1[2] + gA1);
+                st_global(th.gC + v1, q1[3] + gC1);
             }
         }
         rb[xi] = lp;

@@ -260,6 +260,21 @@ template <bool THERMAL, typename S, typename C> struct NlRow {
         }
         return o;
     }
+    // load(x) of the state of step g, row y, without constructing it: both
+    // voxel rows are that of heat slice h(max(g - 1, 0)) (at g = 0 the
+    // first call does not run and row0 = row1)
+    static __device__ __forceinline__ Vox load_at(const MixedInst& I, int g, int y, int x) {
+        Vox o{};
+        if constexpr (THERMAL) {
+            const size_t v = (size_t)heat_slice(max(g - 1, 0), I.nz, I.nzh) * ((size_t)I.nx * I.ny) +
+                             (size_t)y * I.nx + x;
+            o.n = ld_global(I.nn + v);
+            o.dkb = ld_global(I.dkbase + v);
+            o.dkv = ld_global(I.dk + v);
+            o.q = ld_global(I.Q + v);
+        }
+        return o;
+    }
 
     // one point, its heat-grid values loaded in place
     __device__ __forceinline__ void point(C& p, C& s, int x, size_t i) const { point_t<false>(p, s, x, i, nullptr); }
@@ -476,6 +491,14 @@ __global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const Mix
     S* const Sf = static_cast<S*>(I.S);
     // per-slice trace: the state before the SSFM step of slice g
     const bool trace = I.trP && g <= nz - 1 && g % I.trace_every == 0;
+    const int r = t / NX, x = t % NX;
+    using Nl = NlRow<THERMAL, S, C>;
+    // THERMAL from 384 (FP64 arithmetic) or 512 points (FP32): the point's
+    // heat-grid loads are issued after the field loads and are in flight
+    // during the row IFFT (at smaller sizes the registers they hold there
+    // would cost a workgroup per CU)
+    constexpr bool early = THERMAL && NX >= (sizeof(C) == 16 ? 384 : 512);
+    typename Nl::Vox vx{};
     // 1. rows in, spatial (row IFFT unless g == 0), into LDS
     {
         const bool act = gi < 2 * ROWS;
@@ -486,6 +509,9 @@ __global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const Mix
         if (act) {
 #pragma unroll
             for (int q = 0; q < 8; q++) v[q] = from_store<C>(F[base + q * TPR]);
+        }
+        if constexpr (early) vx = Nl::load_at(I, g, blockIdx.x * ROWS + r, x);
+        if (act) {
             if (trace && g == 0) {
                 S* const T = static_cast<S*>(f ? I.trS : I.trP);
                 const uint32_t key = store_key(3, f, g);
@@ -503,11 +529,11 @@ __global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const Mix
     __syncthreads();
     // 2. one point per thread
     {
-        const int r = t / NX, x = t % NX;
         C* rb = lds + 2 * r * LD;
-        const NlRow<THERMAL, S, C> nl(I, g, blockIdx.x * ROWS + r, trace);
+        const Nl nl(I, g, blockIdx.x * ROWS + r, trace);
         C p = rb[fpad(x)], sv = rb[LD + fpad(x)];
-        nl.point(p, sv, x, row0 + (size_t)r * NX + x);
+        if constexpr (early) nl.point(p, sv, x, row0 + (size_t)r * NX + x, vx);
+        else nl.point(p, sv, x, row0 + (size_t)r * NX + x);
         rb[fpad(x)] = p;
         rb[LD + fpad(x)] = sv;
     }
