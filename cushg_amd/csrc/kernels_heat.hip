@@ -66,30 +66,39 @@ __device__ __forceinline__ ZRange zchunk(const HeatCoef& c) {
 }
 
 // Walks the z range [z0, z1) of the column of a non-Dirichlet node (x, y),
-// calling f(z, index, F(T), F(T) - T).
+// calling f(z, index, F(T), F(T) - T).  The loads of the next node (its x
+// and y neighbours and q, and T two planes up) are issued before this
+// node's arithmetic, so that a node waits for no memory latency of its own.
+// Neighbours outside the crystal, which jacobi_vals does not use, are read
+// at the node itself (a valid address, no branch).
 template <typename Fn>
 __device__ __forceinline__ void jacobi_column(const HeatCoef& c, const double* __restrict__ T,
                                               const float* __restrict__ Q, int x, int y, ZRange zr, Fn&& f) {
     const size_t sy = (size_t)c.nx, sz = (size_t)c.nx * c.ny;
     size_t i = (size_t)zr.z0 * sz + (size_t)y * sy + x;
-    const bool hx0 = x > 0, hx1 = x < c.nx - 1, hy1 = y < c.ny - 1;
     if (zr.z0 >= zr.z1) return;
-    double cur = T[i], nxt = zr.z0 + 1 < c.nz ? T[i + sz] : 0.0, prv = zr.z0 > 0 ? T[i - sz] : 0.0;
+    const size_t dxm = x > 0 ? 1 : 0, dxp = x < c.nx - 1 ? 1 : 0, dyp = y < c.ny - 1 ? sy : 0;
+    struct Plane {
+        double xm, xp, ym, yp;
+        float q;
+    };
+    auto plane = [&](size_t j) {
+        return Plane{T[j - dxm], T[j + dxp], T[j - sy], T[j + dyp], Q[j]};
+    };
+    double cur = T[i];
+    double nxt = T[zr.z0 + 1 < c.nz ? i + sz : i];
+    double prv = T[zr.z0 > 0 ? i - sz : i];
+    Plane pc = plane(i);
     for (int z = zr.z0; z < zr.z1; z++, i += sz) {
-        Nb t;
-        t.c = cur;
-        t.zm = prv;
-        t.zp = nxt;
-        t.xm = hx0 ? T[i - 1] : 0.0;
-        t.xp = hx1 ? T[i + 1] : 0.0;
-        t.ym = T[i - sy];
-        t.yp = hy1 ? T[i + sy] : 0.0;
-        const float q = Q[i];
-        const double v = jacobi_vals(c, x, y, z, t, q);
+        const Plane pn = plane(z + 1 < zr.z1 ? i + sz : i);
+        const double up2 = T[z + 2 < c.nz ? i + 2 * sz : i];
+        const Nb t{cur, pc.xm, pc.xp, pc.ym, pc.yp, prv, nxt};
+        const double v = jacobi_vals(c, x, y, z, t, pc.q);
         f(z, i, v, v - cur);
         prv = cur;
         cur = nxt;
-        nxt = z + 2 < c.nz ? T[i + 2 * sz] : 0.0;
+        nxt = up2;
+        pc = pn;
     }
 }
 
@@ -358,9 +367,14 @@ __global__ __launch_bounds__(256) void k_heat_dk(const double* __restrict__ T, d
     if (i >= n2) return;
     const double ls = 0.5 * lp;
     double cum = 0.0;
+    // T of the next two slices in flight during this slice's arithmetic
+    // (past the last slice, the last one is read again: no branch)
+    double t1 = T[i], t2 = T[(size_t)min(1, nzh - 1) * n2 + i];
     for (int h = 0; h < nzh; h++) {
         const size_t v = (size_t)h * n2 + i;
-        const double t = T[v];
+        const double t = t1;
+        t1 = t2;
+        t2 = T[(size_t)min(h + 2, nzh - 1) * n2 + i];
         const double np = refr_index<double>(crystal, lp, t);
         const double ns = refr_index<double>(crystal, ls, t);
         const double d = t - 25.0;
