@@ -407,23 +407,28 @@ template <typename C> struct RowWaves { static constexpr int W = sizeof(C) == 16
 template <int NX, bool THERMAL, int ROWS, typename S, typename C>
 __global__ __launch_bounds__(NX / 8 * ROWS) __attribute__((amdgpu_waves_per_eu(RowWaves<C>::W)))
 void k_rowpass(const MixedInst* __restrict__ insts, Step gs, const double2* __restrict__ tw) {
-    const int g = gs.get();
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
     __shared__ C lds[RowLds<C>::BUFS * ROWS * LD];
     const MixedInst& I = insts[blockIdx.z];
-    const int nz = I.nz;
+    // the descriptor fields and the step are read before any check, so that
+    // the field loads wait for one round of scalar loads, not one per check
+    const int nz = I.nz, ny = I.ny;
+    void* const Pv = I.P;
+    void* const Sv = I.S;
+    const int g = gs.get();
+    asm volatile("" ::"s"(Pv), "s"(Sv));  // keeps the pointer loads ahead of the checks
     if (g > nz) return;
     const int y = blockIdx.x * ROWS + threadIdx.y;
-    if (blockIdx.x * ROWS >= I.ny) return;
+    if (blockIdx.x * ROWS >= ny) return;
     const int tp = threadIdx.x;
     C* my = lds + threadIdx.y * LD;  // pump buffer; with two buffers the SH one is ROWS*LD further
     const size_t base = (size_t)y * NX + tp;
     C f[2][8];  // f[0] pump, f[1] SH: transformed together (shared barriers)
     C(&p)[8] = f[0];
     C(&s)[8] = f[1];
-    S* const P = static_cast<S*>(I.P);
-    S* const Sf = static_cast<S*>(I.S);
+    S* const P = static_cast<S*>(Pv);
+    S* const Sf = static_cast<S*>(Sv);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         p[q] = from_store<C>(P[base + q * TPR]);
@@ -480,15 +485,20 @@ __global__ __launch_bounds__(NX * wide_rows<NX>()) void k_rowpass_wide(const Mix
     constexpr int TPR = NX / 8;
     constexpr int LD = FftShape<NX>::LDS;
     __shared__ C lds[2 * ROWS * LD];
-    const int g = gs.get();
     const MixedInst& I = insts[blockIdx.z];
-    const int nz = I.nz;
+    // the descriptor fields and the step are read before any check, so that
+    // the field loads wait for one round of scalar loads, not one per check
+    const int nz = I.nz, ny = I.ny;
+    void* const Pv = I.P;
+    void* const Sv = I.S;
+    const int g = gs.get();
+    asm volatile("" ::"s"(Pv), "s"(Sv));  // keeps the pointer loads ahead of the checks
     if (g > nz) return;
-    if (blockIdx.x * ROWS >= I.ny) return;
+    if (blockIdx.x * ROWS >= ny) return;
     const int t = threadIdx.x, gi = t / TPR, tp = t % TPR;
     const size_t row0 = (size_t)blockIdx.x * ROWS * NX;
-    S* const P = static_cast<S*>(I.P);
-    S* const Sf = static_cast<S*>(I.S);
+    S* const P = static_cast<S*>(Pv);
+    S* const Sf = static_cast<S*>(Sv);
     // per-slice trace: the state before the SSFM step of slice g
     const bool trace = I.trP && g <= nz - 1 && g % I.trace_every == 0;
     const int r = t / NX, x = t % NX;
@@ -573,14 +583,21 @@ template <typename C> struct ColWaves { static constexpr int W = sizeof(C) == 16
 template <int NY, int COLS, typename S, typename C>
 __global__ __launch_bounds__(NY / 8 * COLS) __attribute__((amdgpu_waves_per_eu(ColWaves<C>::W)))
 void k_colpass(const MixedInst* __restrict__ insts, Step gs, const double2* __restrict__ tw) {
-    const int g = gs.get();
     using R = re_t<C>;
     constexpr int TPC = NY / 8;
     constexpr int LD = FftShape<NY>::LDS;
     __shared__ C lds[COLS * LD];
     const MixedInst& I = insts[blockIdx.z];
-    if (g < 0 || g >= I.nz) return;  // step -1: a lagging launch before the first step
-    const int nx = I.nx;
+    // the descriptor fields and the step are read before any check, so that
+    // the field loads wait for one round of scalar loads, not one per check
+    const bool sh = blockIdx.y != 0;
+    const int nz = I.nz, nx = I.nx;
+    void* const Fv = sh ? I.S : I.P;
+    const double2* const ex = sh ? I.exS : I.exP;
+    const double2* const ey = sh ? I.eyS : I.eyP;
+    const int g = gs.get();
+    asm volatile("" ::"s"(Fv), "s"(ex), "s"(ey));  // keeps the pointer loads ahead of the checks
+    if (g < 0 || g >= nz) return;  // step -1: a lagging launch before the first step
     // A workgroup reads COLS*8 bytes of each 128-byte row segment; the K
     // column groups sharing a segment are placed on one XCD (workgroups are
     // dealt to the 8 XCDs round-robin by linear id, so ids 8 apart share an
@@ -595,10 +612,7 @@ void k_colpass(const MixedInst* __restrict__ insts, Step gs, const double2* __re
     if (x0 >= nx) return;
     const int c = threadIdx.x, tp = threadIdx.y;
     const int x = x0 + c;
-    const bool sh = blockIdx.y != 0;
-    S* F = static_cast<S*>(sh ? I.S : I.P);
-    const double2* ex = sh ? I.exS : I.exP;
-    const double2* ey = sh ? I.eyS : I.eyP;
+    S* F = static_cast<S*>(Fv);
     C* my = lds + c * LD;
     C v[8];
 #pragma unroll
