@@ -136,3 +136,35 @@ def test_graph_replay_is_bit_identical(precision, thermal):
         assert r1.eta == r0.eta and np.array_equal(o1["signal"], o0["signal"]) and seen
     finally:
         h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [abi.PREC_FP32_SPLIT, abi.PREC_FP32_MIXED, abi.PREC_FP32_STORAGE])
+@pytest.mark.parametrize("n", [256, 384, 512])
+def test_thermal_row_pass_load_orders_agree(precision, n):
+    """The THERMAL row pass reads a point's heat-grid values (n_p, n_s, the dk
+    integral base and slope, the heat source) at different times in its two
+    forms: the FP32-arithmetic one-row form one point ahead of the
+    arithmetic (from 192 points), the wide form with the field loads, before
+    the row IFFT (from 512 points in FP32 arithmetic, 384 in FP64), and both in
+    place otherwise.  Every order must give the same bits, including at a heat
+    slab boundary inside the march and on the first slice of a slab (where the
+    heat source read ahead is overwritten unread)."""
+    h = S.Handle(0)
+    try:
+        ps = []
+        for i in range(2):
+            p = configs.scaled(configs.ppln512_thermal(), n=n, nz=14 + 3 * i, nz_heat=4 + i)
+            p.precision = precision
+            p.max_outer = 3
+            p.waist_um += 4.0 * i
+            ps.append(p)
+        print(f"case row pass load orders n={n} precision={precision}", flush=True)
+        h.set_plan(n, abi.PLAN_ROWS_WIDE, 0, precision)
+        wide = run_all(h, ps)
+        assert all(r.peak_dT_K > 0 for r in wide[0])
+        for rows in (1, 2):
+            h.set_plan(n, rows, 0, precision)
+            same(run_all(h, ps), wide)
+    finally:
+        h.close()
