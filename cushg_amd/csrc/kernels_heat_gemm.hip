@@ -130,8 +130,9 @@ __device__ __forceinline__ void stage_store(const Stage<KINDB>& st, float su, in
     }
 }
 
+// kind B fits three waves per SIMD without spilling; kind A does not
 template <bool KINDB>
-__global__ __launch_bounds__(256) void k_heat_gemm(HeatGemm g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KINDB ? 3 : 1))) void k_heat_gemm(HeatGemm g) {
     __shared__ __align__(16) _Float16 Rh[kTI * kLDK], Rl[kTI * kLDK], Sh[kTJ * kLDK], Sl[kTJ * kLDK];
     __shared__ float wmax[4];
     const int b = blockIdx.z;
