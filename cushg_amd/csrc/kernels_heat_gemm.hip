@@ -141,8 +141,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KINDB ? 3 :
     const _Float16* qh = g.qh + (size_t)b * g.sq;
     const _Float16* ql = g.ql + (size_t)b * g.sq;
     const long I = KINDB ? g.nbig : g.n, J = KINDB ? g.n : g.nbig;
-    const long i0 = (long)(KINDB ? blockIdx.x : blockIdx.y) * kTI;
-    const long j0 = (long)(KINDB ? blockIdx.y : blockIdx.x) * kTJ;
+    // Tile order (grid x: data tiles, y: eigenbasis tiles).  The gy tiles
+    // that share a data tile are placed 8 apart in the dispatch order: the
+    // same XCD (workgroups are dealt to the 8 XCDs round-robin) at about the
+    // same time, so the data tile comes from HBM once and from that XCD's L2
+    // for the others.  A bijection of the grid: groups of 8 data tiles, then
+    // the remaining gx % 8 in plain order.
+    int dt = blockIdx.x, et = blockIdx.y;
+    {
+        const int gx = gridDim.x, gy = gridDim.y;
+        if (gy > 1) {
+            const int L = blockIdx.x + gx * blockIdx.y;
+            const int full = gx / 8 * 8, head = full * gy;
+            if (L < head) {
+                const int chunk = L / (8 * gy), r = L - chunk * 8 * gy;
+                dt = chunk * 8 + (r & 7);
+                et = r >> 3;
+            } else {
+                const int l = L - head, rem = gx - full;
+                dt = full + l % rem;
+                et = l / rem;
+            }
+        }
+    }
+    const long i0 = (long)(KINDB ? dt : et) * kTI;
+    const long j0 = (long)(KINDB ? et : dt) * kTJ;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wi = w >> 1, wj = w & 1;
     const float mx = __uint_as_float(*g.umax);
